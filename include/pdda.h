@@ -1,0 +1,147 @@
+/*
+ * pdda.h - C ABI of libpdda.so, the gfx950 DDA scattering solver.
+ *
+ * Plain pointers and sizes only. Every `void*` operand is a device pointer (HIP
+ * global memory) unless documented as host memory; `stream` is a hipStream_t (NULL =
+ * the null stream). Precision `prec`: 0 = complex64 (float2), 1 = complex128 (double2).
+ *
+ * Data formats (identical to the reference's on-device formats):
+ *   box vector      flat 3N complex, C order (x, y, z, component), component fastest,
+ *                   N = nx*ny*nz (Gpbicgstab.py:743, 933).
+ *   reduced table   (hx+1, hy+1, hz+1, 6) complex, components xx,xy,xz,yy,yz,zz
+ *                   (ddaforgit.py:972-977), holding FFT(-A) on the (2hx, 2hy, 2hz) grid
+ *                   (ddaforgit.py:1076); the full grid is recovered with the mirror rule
+ *                   of Gpbicgstab.py:192-211.
+ *   inv_alpha       per site (N) complex, 0 off target (ddaforgit.py:1105).
+ *   mask            per site (N) uint8, nonzero on target (ddaforgit.py:1108).
+ *
+ * Every entry point returns 0 on success or a negative PDDA_ERR_* code; the message
+ * is available from pdda_last_error() on the calling thread.
+ */
+#ifndef PDDA_H
+#define PDDA_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PDDA_ABI_VERSION 1
+
+#define PDDA_OK 0
+#define PDDA_ERR_ARG (-1)
+#define PDDA_ERR_SIZE (-2)
+#define PDDA_ERR_HIP (-3)
+#define PDDA_ERR_OOM (-4)
+#define PDDA_ERR_STATE (-5)
+
+/* convolution epilogues (pdda_conv_apply `mode`) */
+#define PDDA_MODE_FIELD 0    /* out = conv(x)                            ExcitedField.py:129-195 */
+#define PDDA_MODE_PRECOND 1  /* out = mask * conv(x)                     Gpbicgstab.py:633-668 */
+#define PDDA_MODE_INTERACT 2 /* out = inv_alpha x - mask conv(x)         Gpbicgstab.py:828-830 */
+#define PDDA_MODE_RESIDUAL 3 /* out = b - (inv_alpha x - mask conv(x))   Gpbicgstab.py:1059-1067 */
+
+/* solver slot status */
+#define PDDA_ST_IDLE 0
+#define PDDA_ST_RUNNING 1
+#define PDDA_ST_CONVERGED 2
+#define PDDA_ST_FAILED 3 /* NaN residual or stagnation window */
+#define PDDA_ST_MAXITER 4
+
+typedef struct pdda_conv pdda_conv;
+typedef struct pdda_solver pdda_solver;
+typedef struct pdda_mixed pdda_mixed;
+
+typedef struct pdda_slot_status {
+  int status;     /* PDDA_ST_* */
+  int iters;      /* iterations run (stop index + 1; max_iter on PDDA_ST_MAXITER) */
+  int resets;
+  int flags;
+  double initial_norm, current_norm, lowest_norm, first_norm;
+} pdda_slot_status;
+
+typedef struct pdda_mixed_report {
+  int outer;          /* FP64 refinement steps */
+  int inner_iters;    /* complex64 GPBiCG iterations summed over the inner solves */
+  double final_rel;   /* FP64 ||b - A x|| / ||b|| at exit */
+  double reserved;
+} pdda_mixed_report;
+
+int pdda_abi_version(void);
+const char* pdda_last_error(void);
+/* 1 if an FFT length is supported (products of 2,3,5,7,11,13, at most 1024). */
+int pdda_fft_length_supported(int L);
+
+/* Reduced interaction table FFT(-A) on the 2n grid, built in FP64 on the device.
+ * Replaces generate_interaction_row (ddaforgit.py:1047-1080) with reduced=True;
+ * out_c128 (required) and out_c64 (optional) hold (nx+1)(ny+1)(nz+1)*6 elements. */
+int pdda_green_build(int nx, int ny, int nz, double k, double d, void* out_c128, void* out_c64, void* stream);
+
+/* Circulant preconditioner table M(q) = (gamma I - G(q))^-1 from a complex128 reduced
+ * table on the (2hx,2hy,2hz) grid. Takes the place of the absent
+ * pre_preconditioner/circulant_approximation (ddaforgit.py:1118-1136). */
+int pdda_precond_build(int hx, int hy, int hz, double gamma_re, double gamma_im, const void* ghat_c128,
+                       void* out_c128, void* out_c64, void* stream);
+
+/* Convolution plan for a box (nx,ny,nz) and a table on the (2hx,2hy,2hz) grid, for
+ * `nslots` box vectors stored back to back. Owns its FFT workspace. */
+int pdda_conv_create(pdda_conv** out, int prec, int nx, int ny, int nz, int hx, int hy, int hz, int nslots);
+int pdda_conv_destroy(pdda_conv* c);
+long long pdda_conv_workspace_bytes(const pdda_conv* c);
+/* One convolution per slot. Replaces fft_matvec + interaction_mult
+ * (Gpbicgstab.py:742-772, 828-830), fft_matvec3 (:633-668) and the product of
+ * E_field_solver (ExcitedField.py:155-195). table_stride / site_stride: elements
+ * between the per-slot tables / site arrays (0 = shared by all slots). */
+int pdda_conv_apply(pdda_conv* c, int mode, const void* x, void* out, const void* table, long long table_stride,
+                    const void* inv_alpha, const void* mask, long long site_stride, const void* b, void* stream);
+
+/* sum conj(a) b over n complex elements; result (re, im) to host out2.
+ * Replaces inner_product (Gpbicgstab.py:449-464). */
+int pdda_dot(int prec, const void* a, const void* b, long long n, double* out2, void* stream);
+/* zeta, eta of the GPBiCG step from y, t, s; result (zeta re, im, eta re, im) to host.
+ * Replaces calculate_zeta_eta_twostep (Gpbicgstab.py:360-380). */
+int pdda_zeta_eta(int prec, const void* y, const void* t, const void* s, long long n, double* out4, void* stream);
+
+/* Cross sections (Draine & Flatau 1994) of a box polarisation P for a unit-amplitude
+ * incident field; host out2 = (C_ext, C_abs). The reference has no counterpart. */
+int pdda_cross_sections(int prec, const void* P, const void* E_inc, const void* inv_alpha, long long nsites,
+                        double k, double* out2, void* stream);
+
+/* Device-resident preconditioned GPBiCG for `nslots` right-hand sides sharing one box.
+ * Replaces solve_gpbicgstab (Gpbicgstab.py:879-1201). precon=0 makes the
+ * preconditioner the identity (Gpbicgstab.py:634-635); (px,py,pz) are the table half
+ * sizes of the preconditioner grid (0 = the box sizes). */
+int pdda_solver_create(pdda_solver** out, int prec, int nx, int ny, int nz, int px, int py, int pz, int nslots,
+                       int precon);
+int pdda_solver_destroy(pdda_solver* s);
+int pdda_solver_bind(pdda_solver* s, const void* ghat, long long ghat_stride, const void* mhat,
+                     long long mhat_stride, const void* inv_alpha, const void* mask, long long site_stride);
+/* Loads E_inc for one slot (slot >= 0) or all slots (slot = -1, nslots vectors) and
+ * resets the slot state. target: relative recursive-residual target (Gpbicgstab.py:888);
+ * ratio: reset ratio (0 disables resets); stagnation: non-improving iterations before
+ * failure (the reference hard-codes 5000, Gpbicgstab.py:1105). */
+int pdda_solver_load(pdda_solver* s, int slot, const void* E_inc, double target, int max_iter, double ratio,
+                     int stagnation, void* stream);
+/* Enqueues n_iter iterations for every running slot; no host synchronisation. */
+int pdda_solver_iterate(pdda_solver* s, int n_iter, void* stream);
+/* Synchronises the stream and copies the slot states to host `out[nslots]`. */
+int pdda_solver_status(pdda_solver* s, pdda_slot_status* out, int nslots, void* stream);
+/* P = M x (converged) or M x_best (otherwise) for every slot (Gpbicgstab.py:1162-1201). */
+int pdda_solver_result(pdda_solver* s, void* P_out, void* stream);
+
+/* complex64 GPBiCG inner solves with FP64 true-residual refinement. */
+int pdda_mixed_create(pdda_mixed** out, int nx, int ny, int nz, int px, int py, int pz, int nslots, int precon);
+int pdda_mixed_destroy(pdda_mixed* m);
+int pdda_mixed_bind(pdda_mixed* m, const void* ghat_c128, const void* ghat_c64, long long ghat_stride,
+                    const void* mhat_c64, long long mhat_stride, const void* inv_alpha_c128,
+                    const void* inv_alpha_c64, const void* mask, long long site_stride);
+/* Solves to FP64 relative residual `tol`; E_inc complex128, nslots vectors. */
+int pdda_mixed_solve(pdda_mixed* m, const void* E_inc_c128, double tol, double inner_target, int max_outer,
+                     int max_inner, double ratio, int stagnation, int check_every, void* stream);
+/* Copies the FP64 solutions (nslots box vectors, complex128) and per-slot reports. */
+int pdda_mixed_result(pdda_mixed* m, void* P_out_c128, pdda_mixed_report* rep, int nslots, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PDDA_H */

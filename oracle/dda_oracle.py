@@ -1,0 +1,708 @@
+"""CPU oracle for the DDA solve path: a NumPy/SciPy complex128 restatement.
+
+TEST INFRASTRUCTURE ONLY. This module is the checker that the HIP path is compared
+against. Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` may import it. The product package ``precondda_amd`` never imports it.
+
+Every function cites the reference file:line it restates (reference =
+Toastlovesjam/preconDDA: ``ddaforgit.py``, ``Gpbicgstab.py``, ``ExcitedField.py``).
+The reference GPU code is complex64 CuPy/cuFFT; its FP64 solver module is absent from
+the reference repository, so "the reference solver in FP64" is this restatement run in
+complex128.
+
+How each function is pinned (tests/test_oracle.py):
+  calculate_alpha        closed forms: CM limit at k=0, RR special case, real-m
+                         identity Im(1/alpha) = -(2/3) k^3 for RR and LDR.
+  inside tests/marking   brute-force point tests, exact dipole counts of the five
+                         benchmark targets.
+  greens_real            dyadic Green's function from finite differences of the
+                         scalar Green's function, (k^2 I + grad grad) e^{ikr}/r.
+  generate_interaction_row / expand_reduced
+                         full unreduced FFT of the embedded tensor (mirror symmetry).
+  fft_matvec/interaction_mult
+                         explicitly assembled dense DDA matrix (direct pair sums).
+  solve_gpbicgstab       numpy.linalg.solve on the dense matrix.
+  cross_sections         energy conservation C_ext = C_abs + C_sca with C_sca from an
+                         independent far-field quadrature; C_abs = 0 for real m.
+  E_field_solver         E_inc + E_sca = inv_alpha * P at occupied sites (exact solve).
+  build_preconditioner   parity unpinned (the reference's builder is absent from the
+                         reference repository); pinned only by invariants: the exact
+                         circulant case converges in one step, precon on/off agree.
+"""
+from __future__ import annotations
+
+import math
+import os
+
+import numpy as np
+
+try:  # scipy.fft is faster and takes a worker count; numpy.fft is the fallback.
+    import scipy.fft as _sfft
+
+    def _fft(a, n=None, axis=-1):
+        return _sfft.fft(a, n=n, axis=axis, workers=_WORKERS)
+
+    def _ifft(a, n=None, axis=-1):
+        return _sfft.ifft(a, n=n, axis=axis, workers=_WORKERS)
+
+    def _fftn(a, axes):
+        return _sfft.fftn(a, axes=axes, workers=_WORKERS)
+
+except ImportError:  # pragma: no cover
+    def _fft(a, n=None, axis=-1):
+        return np.fft.fft(a, n=n, axis=axis)
+
+    def _ifft(a, n=None, axis=-1):
+        return np.fft.ifft(a, n=n, axis=axis)
+
+    def _fftn(a, axes):
+        return np.fft.fftn(a, axes=axes)
+
+_WORKERS = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else 1
+
+
+def set_workers(n: int) -> None:
+    """Thread count used by scipy.fft in the oracle (the CPU baseline reports it)."""
+    global _WORKERS
+    _WORKERS = max(1, int(n))
+
+
+def get_workers() -> int:
+    return _WORKERS
+
+
+# ----------------------------------------------------------------------------------
+# Polarisability  (ddaforgit.py:403-437)
+# ----------------------------------------------------------------------------------
+LDR_B1 = -1.891531
+LDR_B2 = 0.1648469
+LDR_B3 = -1.7700004
+
+
+def calculate_alpha(refractive_index, lattice_spacing, wave_number,
+                    incident_direction=None, polarization_direction=None, method="LDR"):
+    """CM / RR / LDR polarisability (ddaforgit.py:403-437).
+
+    LDR uses the reference's S = (a.e)^2 (ddaforgit.py:432), which is zero for every
+    transverse plane wave.
+    """
+    n = np.complex128(refractive_index)
+    d = np.float64(lattice_spacing)
+    k = np.float64(wave_number)
+    eps = n * n
+    a_cm = (3.0 * d ** 3 / (4.0 * np.pi)) * (eps - 1.0) / (eps + 2.0)
+    if method == "CM":
+        return a_cm
+    rr = (2.0 / 3.0) * 1j * (k * d) ** 3
+    if method == "RR":
+        return a_cm / (1.0 + (a_cm / d ** 3) * (-rr))
+    if method == "LDR":
+        if incident_direction is None or polarization_direction is None:
+            raise ValueError("incident_direction and polarization_direction required for LDR method")
+        a = np.asarray(incident_direction, dtype=np.float64)
+        e = np.asarray(polarization_direction, dtype=np.float64)
+        S = float(np.dot(a, e)) ** 2
+        ldr = (LDR_B1 + eps * LDR_B2 + eps * LDR_B3 * S) * (k * d) ** 2
+        return a_cm / (1.0 + (a_cm / d ** 3) * (ldr - rr))
+    raise ValueError("method must be 'CM', 'RR', or 'LDR'")
+
+
+# ----------------------------------------------------------------------------------
+# Targets and lattice  (ddaforgit.py:291-400, 440-474, 615-661, 831-857)
+# ----------------------------------------------------------------------------------
+EPS_EDGE = 1e-7
+
+
+def inside_points(shape: dict, pts: np.ndarray) -> np.ndarray:
+    """Occupancy rules of ddaforgit.py:440-474 (sphere, ellipsoid, rectangle)."""
+    c = np.asarray(shape["center"], dtype=np.float64)
+    t = shape["type"]
+    if t == "sphere":  # ddaforgit.py:440-448
+        dist = np.sqrt(((pts - c) ** 2).sum(axis=1))
+        R = float(shape["radius"])
+        return (dist < R) | (np.abs(dist - R) < EPS_EDGE)
+    if t == "ellipsoid":  # ddaforgit.py:451-459
+        ax = np.asarray(shape["semi_axes"], dtype=np.float64)
+        dist = np.sqrt((((pts - c) / ax) ** 2).sum(axis=1))
+        return dist <= 1.0 + EPS_EDGE
+    if t == "rectangle":  # ddaforgit.py:462-474
+        half = np.asarray(shape["dimensions"], dtype=np.float64) / 2.0
+        dd = np.abs(pts - c)
+        inside = np.all(dd <= half, axis=1)
+        on = np.zeros(len(pts), dtype=bool)
+        for a in range(3):
+            o = [b for b in range(3) if b != a]
+            on |= (np.abs(dd[:, a] - half[a]) < EPS_EDGE) & (dd[:, o[0]] <= half[o[0]]) & (dd[:, o[1]] <= half[o[1]])
+        return inside | on
+    raise ValueError(f"unsupported shape type {t}")
+
+
+def _max_extent(shape: dict) -> float:
+    return max(float(shape.get("radius", 0.0)), float(shape.get("height", 0.0)) / 2.0,
+               max(shape.get("semi_axes", [0, 0, 0])), max(shape.get("dimensions", [0, 0, 0])) / 2.0)
+
+
+def snap_centers(shapes: list[dict], d: float) -> list[dict]:
+    """process_shapes centre snapping with ceil (ddaforgit.py:311-317, adjust_all=True)."""
+    out = []
+    for s in shapes:
+        s2 = dict(s)
+        s2["center"] = list(d * np.ceil(np.asarray(s["center"], dtype=np.float64) / d))
+        out.append(s2)
+    return out
+
+
+def grid_extents(shapes: list[dict], d: float):
+    """calculate_grid_extents for sphere/ellipsoid/rectangle (ddaforgit.py:346-400)."""
+    mx = np.zeros(3)
+    for s in shapes:
+        mx = np.maximum(mx, np.asarray(s["center"], dtype=np.float64) + _max_extent(s))
+    mx = mx + d
+    return tuple(int(np.ceil(v / d)) for v in mx)
+
+
+def mark_shapes(grid_shape, d, shapes: list[dict], values: list, voxel_center=True):
+    """alpha grid with later shapes overwriting earlier ones (ddaforgit.py:615-661)."""
+    nx, ny, nz = grid_shape
+    off = 0.5 if voxel_center else 0.0
+    out = np.zeros(grid_shape, dtype=np.complex128)
+    for s, val in zip(shapes, values):
+        mgd = int(float(_max_extent(s) + EPS_EDGE) / d) + 1
+        ci = (np.asarray(s["center"], dtype=np.float64) / d).astype(int)
+        lo = [max(0, ci[a] - mgd) for a in range(3)]
+        hi = [min(n, ci[a] + mgd + 1) for a, n in enumerate((nx, ny, nz))]
+        axes = [(np.arange(lo[a], hi[a]) + off) * d for a in range(3)]
+        X, Y, Z = np.meshgrid(*axes, indexing="ij")
+        pts = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+        ins = inside_points(s, pts).reshape(X.shape)
+        sl = out[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]]
+        sl[ins] = val
+    return out
+
+
+def trim_zero_faces(a: np.ndarray) -> np.ndarray:
+    """Crop empty faces, then pad odd dims to even at the high end (ddaforgit.py:831-857)."""
+    nzm = a != 0
+    xm = nzm.any(axis=(1, 2))
+    ym = nzm.any(axis=(0, 2))
+    zm = nzm.any(axis=(0, 1))
+    if not xm.any():
+        return a[:0, :0, :0]
+    def b(m):
+        idx = np.nonzero(m)[0]
+        return idx[0], idx[-1] + 1
+    (x0, x1), (y0, y1), (z0, z1) = b(xm), b(ym), b(zm)
+    t = a[x0:x1, y0:y1, z0:z1]
+    ns = tuple(s + (s % 2) for s in t.shape)
+    if ns != t.shape:
+        p = np.zeros(ns, dtype=a.dtype)
+        p[:t.shape[0], :t.shape[1], :t.shape[2]] = t
+        return p
+    return t
+
+
+def build_problem(shapes: list[dict], k_direction, E_polarization, wavelength, d=1.0):
+    """ShapeManager + create_alpha_and_n_arrays + plane wave (ddaforgit.py:1591-1645).
+
+    Shapes keep insertion order for marking; priorities are all zero here so the
+    priority sort of process_shapes (ddaforgit.py:297) is the identity.
+    """
+    k = 2.0 * np.pi / wavelength
+    kdir = np.asarray(k_direction, dtype=np.float64)
+    epol = np.asarray(E_polarization, dtype=np.float64)
+    shapes = snap_centers(shapes, d)
+    amap = {}
+    for s in shapes:
+        if s["n"] not in amap:
+            amap[s["n"]] = calculate_alpha(s["n"], d, k, kdir, epol, method="LDR")
+    vals = [amap[s["n"]] for s in shapes]
+    g = grid_extents(shapes, d)
+    alpha = trim_zero_faces(mark_shapes(g, d, shapes, vals))
+    E_inc = plane_wave(alpha.shape, k, kdir, epol, d)
+    return {"alpha_array": alpha, "grid_size": alpha.shape, "E_inc": E_inc, "k": k,
+            "lattice_spacing": d, "k_vec": k * kdir, "k_direction": kdir,
+            "E_polarization": epol, "wavelength": wavelength}
+
+
+def plane_wave(grid, k, kdir, epol, d=1.0):
+    """E_inc[i,j,l,:] = e * exp(i k.(i,j,l) d), integer index, no 0.5 offset (ddaforgit.py:1626-1632)."""
+    kv = k * np.asarray(kdir, dtype=np.float64)
+    axes = [np.arange(0, n) * d for n in grid]
+    X, Y, Z = np.meshgrid(*axes, indexing="ij")
+    ph = np.exp(1j * (kv[0] * X + kv[1] * Y + kv[2] * Z))
+    E = np.zeros((*grid, 3), dtype=np.complex128)
+    for j in range(3):
+        E[..., j] = np.asarray(epol, dtype=np.float64)[j] * ph
+    return E
+
+
+# ----------------------------------------------------------------------------------
+# Interaction tensor  (ddaforgit.py:993-1080)
+# ----------------------------------------------------------------------------------
+COMP_AB = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]  # xx,xy,xz,yy,yz,zz (ddaforgit.py:972-977)
+
+
+def greens_real(nx, ny, nz, k, d):
+    """Real-space tensor A_ab on the 2n-periodic embedding, all six components.
+
+    Restates greens_kernel_double (ddaforgit.py:993-1044): positions folded by -2n d for
+    i >= n, zero at the origin and on the i == n planes,
+    A = e^{ikr}/r [k^2 (r_a r_b - delta_ab) + (ikr - 1)/r^2 (3 r_a r_b - delta_ab)].
+    Returns (2nx, 2ny, 2nz, 6) complex128.
+    """
+    def axis(n):
+        i = np.arange(2 * n)
+        p = i * d - np.where(i >= n, 2 * n * d, 0.0)
+        return i, p
+    ix, px = axis(nx)
+    iy, py = axis(ny)
+    iz, pz = axis(nz)
+    X, Y, Z = np.meshgrid(px, py, pz, indexing="ij")
+    r2 = X * X + Y * Y + Z * Z
+    r2[0, 0, 0] = 1.0
+    r = np.sqrt(r2)
+    u = [X / r, Y / r, Z / r]
+    e = np.exp(1j * k * r) / r
+    f2 = (1j * k * r - 1.0) / r2
+    mult = np.ones_like(r)
+    mult[nx, :, :] = 0.0
+    mult[:, ny, :] = 0.0
+    mult[:, :, nz] = 0.0
+    mult[0, 0, 0] = 0.0
+    out = np.empty((2 * nx, 2 * ny, 2 * nz, 6), dtype=np.complex128)
+    for c, (a, b) in enumerate(COMP_AB):
+        nc = u[a] * u[b]
+        dg = 1.0 if a == b else 0.0
+        out[..., c] = mult * e * (k * k * (nc - dg) + (3.0 * nc - dg) * f2)
+    return out
+
+
+def generate_interaction_row(nx, ny, nz, k, d, reduced=True):
+    """FFT(-A) per component, cropped to (nx+1, ny+1, nz+1, 6) (ddaforgit.py:1047-1080)."""
+    A = greens_real(nx, ny, nz, k, d)
+    F = _fftn(-A, axes=(0, 1, 2))
+    if reduced:
+        return np.ascontiguousarray(F[:nx + 1, :ny + 1, :nz + 1, :])
+    return F
+
+
+def expand_reduced(G: np.ndarray) -> np.ndarray:
+    """Full (2hx, 2hy, 2hz, 3, 3) tensor from the reduced 6-component table.
+
+    Mirror rule of Gpbicgstab.py:41-103 / 167-237: m = k <= h ? k : 2h - k, and the
+    off-diagonal ab entry flips sign iff exactly one of the axes a, b is mirrored.
+    """
+    hx, hy, hz = (s - 1 for s in G.shape[:3])
+    def mir(h):
+        k = np.arange(2 * h)
+        return np.where(k > h, 2 * h - k, k), (k > h)
+    mx, fx = mir(hx)
+    my, fy = mir(hy)
+    mz, fz = mir(hz)
+    sub = G[np.ix_(mx, my, mz)]  # (2hx,2hy,2hz,6)
+    FX = fx[:, None, None]
+    FY = fy[None, :, None]
+    FZ = fz[None, None, :]
+    sxy = np.where(FX != FY, -1.0, 1.0)
+    sxz = np.where(FX != FZ, -1.0, 1.0)
+    syz = np.where(FY != FZ, -1.0, 1.0)
+    T = np.empty(sub.shape[:3] + (3, 3), dtype=G.dtype)
+    T[..., 0, 0] = sub[..., 0]
+    T[..., 1, 1] = sub[..., 3]
+    T[..., 2, 2] = sub[..., 5]
+    T[..., 0, 1] = T[..., 1, 0] = sxy * sub[..., 1]
+    T[..., 0, 2] = T[..., 2, 0] = sxz * sub[..., 2]
+    T[..., 1, 2] = T[..., 2, 1] = syz * sub[..., 4]
+    return T
+
+
+def _sym_apply(T, Y):
+    """y_a = sum_b T_ab Y_b on the full grid."""
+    return np.einsum("...ab,...b->...a", T, Y, optimize=True)
+
+
+def conv_reduced(x: np.ndarray, G: np.ndarray, out_shape=None) -> np.ndarray:
+    """IFFT3(G_sym . FFT3(pad(x)))[:nx,:ny,:nz] on the grid (2hx, 2hy, 2hz).
+
+    Pruned per-axis form of the reference's chunked convolution
+    (Gpbicgstab.py:742-772 for the interaction tensor, :633-668 for the
+    preconditioner): zero padding is expressed through the FFT length, and the inverse
+    transforms are cropped axis by axis. Mathematically identical to the full 3-D form.
+    """
+    nx, ny, nz = x.shape[:3]
+    Lx, Ly, Lz = (2 * (s - 1) for s in G.shape[:3])
+    y = _fft(x, n=Lz, axis=2)
+    y = _fft(y, n=Ly, axis=1)
+    y = _fft(y, n=Lx, axis=0)
+    y = _sym_apply(expand_reduced(G), y)
+    y = _ifft(y, axis=0)[:nx]
+    y = _ifft(y, axis=1)[:, :ny]
+    y = _ifft(y, axis=2)[:, :, :nz]
+    return y
+
+
+def fft_matvec(x, G, mask, grid):
+    """mask * IFFT(Ghat . FFT(pad x)) on the box (Gpbicgstab.py:742-772)."""
+    nx, ny, nz = grid
+    X = np.asarray(x).reshape(nx, ny, nz, 3)
+    return (conv_reduced(X, G) * np.asarray(mask, dtype=np.float64).reshape(nx, ny, nz, 1)).ravel()
+
+
+def interaction_mult(x, G, inv_alpha3, mask, grid):
+    """v = inv_alpha * x - fft_matvec(x) (Gpbicgstab.py:828-830, :473-475)."""
+    return inv_alpha3 * np.asarray(x).ravel() - fft_matvec(x, G, mask, grid)
+
+
+def fft_matvec3(x, precond, mask, grid, precon=True):
+    """Preconditioner apply on the full 2P grid, crop, mask (Gpbicgstab.py:633-668)."""
+    if not precon:
+        return np.asarray(x).ravel().copy()
+    nx, ny, nz = grid
+    X = np.asarray(x).reshape(nx, ny, nz, 3)
+    return (conv_reduced(X, precond) * np.asarray(mask, dtype=np.float64).reshape(nx, ny, nz, 1)).ravel()
+
+
+def E_field_solver(P, G, grid):
+    """E_sca = IFFT(Ghat . FFT(P)) on all sites, no mask (ExcitedField.py:129-195)."""
+    nx, ny, nz = grid
+    P = np.asarray(P)
+    if P.size != nx * ny * nz * 3:
+        raise ValueError(f"Polarization size {P.size} doesn't match expected size {nx * ny * nz * 3}")
+    return conv_reduced(P.reshape(nx, ny, nz, 3), G)
+
+
+# ----------------------------------------------------------------------------------
+# Preconditioner builder (the reference's builder is absent; parity unpinned)
+# ----------------------------------------------------------------------------------
+def build_preconditioner(grid, k, d, inv_alpha, refract_mult=1.0, expansion=None):
+    """Circulant preconditioner in the reduced mirror format on the 2P grid.
+
+    The reference calls pre_preconditioner / circulant_approximation
+    (ddaforgit.py:1118-1136) whose source is not in the reference repository, and
+    applies the result with fft_matvec3 (Gpbicgstab.py:633-668). This build defines
+    M_hat(q) = (gamma I - Ghat_P(q))^-1 per frequency, where Ghat_P is the reduced FFT of
+    -A on the 2P grid and gamma = refract_mult * mean(inv_alpha over occupied sites):
+    the exact inverse of the fully occupied periodic operator.
+    """
+    nx, ny, nz = grid
+    Px, Py, Pz = expansion if expansion is not None else (nx, ny, nz)
+    ia = np.asarray(inv_alpha).reshape(-1)
+    occ = ia != 0
+    gamma = complex(refract_mult) * (ia[occ].mean() if occ.any() else 1.0)
+    Gp = generate_interaction_row(Px, Py, Pz, k, d)
+    T = np.zeros(Gp.shape[:3] + (3, 3), dtype=np.complex128)
+    T[..., 0, 0] = gamma - Gp[..., 0]
+    T[..., 1, 1] = gamma - Gp[..., 3]
+    T[..., 2, 2] = gamma - Gp[..., 5]
+    T[..., 0, 1] = T[..., 1, 0] = -Gp[..., 1]
+    T[..., 0, 2] = T[..., 2, 0] = -Gp[..., 2]
+    T[..., 1, 2] = T[..., 2, 1] = -Gp[..., 4]
+    Ti = np.linalg.inv(T)
+    M = np.empty_like(Gp)
+    for c, (a, b) in enumerate(COMP_AB):
+        M[..., c] = Ti[..., a, b]
+    return M, gamma
+
+
+# ----------------------------------------------------------------------------------
+# Krylov primitives and the solver  (Gpbicgstab.py:239-528, 879-1201)
+# ----------------------------------------------------------------------------------
+def inner_product(a, b):
+    """sum conj(a) b (Gpbicgstab.py:383-464)."""
+    return np.vdot(a, b)
+
+
+def calculate_zeta_eta(y, t, s):
+    """Six conj-left sums and the 2x2 solve (Gpbicgstab.py:239-380)."""
+    yy = np.vdot(y, y)
+    tt = np.vdot(t, t)
+    yt = np.vdot(y, t)
+    ty = np.vdot(t, y)
+    ys = np.vdot(y, s)
+    ts = np.vdot(t, s)
+    den = tt * yy - yt * ty
+    zeta = (yy * ts - ys * ty) / den
+    eta = (tt * ys - yt * ts) / den
+    return zeta, eta
+
+
+def solve_gpbicgstab(grid_size, inv_alpha, interaction_matrix, preconditioner, E_inc, mask,
+                     ratio, max_iter, target=1e-5, precon=True, dtype=np.complex128,
+                     stagnation=5000, trace=None):
+    """Step-by-step restatement of solve_gpbicgstab (Gpbicgstab.py:879-1201).
+
+    Right-preconditioned GPBiCG with the reference's quirks: zeta-only first step and
+    post-reset step (:1038-1050), true-residual replacement at i == 1 (:1059-1067),
+    forced reset at i == 2 and ratio resets (:1126-1137), a reset recomputes r only if
+    i % 100 != 0 (:1142), rho_1 is taken before the reset (:1121), best-iterate return
+    on failure (:1162-1194). ``dtype`` complex128 is the FP64 plain oracle.
+    Returns (iters, P (nx,ny,nz,3), lowest/initial, first/initial).
+    """
+    nx, ny, nz = grid_size
+    rdt = np.float64 if dtype == np.complex128 else np.float32
+    m = np.asarray(mask).astype(rdt).reshape(nx, ny, nz, 1)
+    b = (np.asarray(E_inc).astype(dtype) * m).ravel()
+    r0 = b.copy()
+    ia = np.repeat(np.asarray(inv_alpha).astype(dtype).ravel(), 3)
+    G = np.asarray(interaction_matrix).astype(dtype)
+    Mp = np.asarray(preconditioner).astype(dtype) if precon else None
+    mask3 = np.asarray(mask)
+
+    def A(x):
+        return interaction_mult(x, G, ia, mask3, grid_size).astype(dtype)
+
+    def M(x):
+        return fft_matvec3(x, Mp, mask3, grid_size, precon).astype(dtype)
+
+    r0_hat = r0.copy()
+    p = r0.copy()
+    p_current = np.zeros_like(r0)
+    w = np.zeros_like(r0)
+    u = np.zeros_like(r0)
+    z = np.zeros_like(r0)
+    s = np.zeros_like(r0)
+    s_prev = np.zeros_like(r0)
+    initial_norm = float(np.linalg.norm(r0))
+    rho_1 = inner_product(r0_hat, r0)
+    rho_0 = rho_1
+    beta = dtype(0)
+    alpha = dtype(0)
+    zeta = dtype(0)
+    lowest_norm = initial_norm
+    current_norm = lowest_norm
+    previous_lowest = lowest_norm
+    first_norm = 0.0
+    best_p_current = np.zeros_like(r0)
+    was_reset = False
+    last_change = 0
+    resets = 0
+    i = -1
+    for i in range(max_iter):
+        phat = M(p)
+        v = A(phat)
+        alpha = rho_1 / inner_product(r0_hat, v)
+        s = r0 - alpha * v
+        y = s_prev - s - alpha * w
+        shat = M(s)
+        t = A(shat)
+        if i == 0 or was_reset:
+            zeta = inner_product(t, s) / inner_product(t, t)
+            u = zeta * v
+            z = zeta * s
+            r0 = s - zeta * t
+        else:
+            zeta, eta = calculate_zeta_eta(y, t, s)
+            u = zeta * v + eta * (s_prev - r0 + beta * u)
+            z = zeta * r0 + eta * z - alpha * u
+            r0 = s - eta * y - zeta * t
+        p_current = p_current + alpha * p + z
+        if i == 1:
+            r0 = b - A(M(p_current))
+        current_norm = float(np.linalg.norm(r0))
+        if trace is not None:
+            trace.append(current_norm)
+        if i == 0:
+            first_norm = current_norm
+        last_change += 1
+        if (not np.isnan(current_norm)) and (np.isnan(lowest_norm) or current_norm < lowest_norm):
+            last_change = 0
+            lowest_norm = current_norm
+            best_p_current = p_current.copy()
+        if np.isnan(current_norm) or last_change >= stagnation:
+            break
+        elif current_norm < target * initial_norm:
+            break
+        reset = False
+        rho_0 = rho_1
+        rho_1 = inner_product(r0_hat, r0)
+        if ratio != 0:
+            if i == 2 or (i > 2 and current_norm < previous_lowest / ratio):
+                reset = True
+                previous_lowest = current_norm
+                last_change = 0
+        was_reset = False
+        if reset:
+            if i % 100 != 0:
+                r0 = b - A(M(p_current))
+            r0_hat = r0.copy()
+            beta = dtype(0)
+            w = np.zeros_like(w)
+            s_prev = s.copy()
+            was_reset = True
+            p = r0.copy()
+            resets += 1
+        else:
+            beta = alpha / zeta * rho_1 / rho_0
+            w = t + beta * v
+            s_prev = s.copy()
+            p = r0 + beta * (p - u)
+    if np.isnan(current_norm) or last_change >= stagnation:
+        P = M(best_p_current)
+        return -(i + 1), P.reshape(nx, ny, nz, 3), lowest_norm / initial_norm, first_norm / initial_norm
+    elif i < max_iter - 1 and current_norm >= target * initial_norm:
+        P = M(best_p_current)
+        return -(i + 1), P.reshape(nx, ny, nz, 3), lowest_norm / initial_norm, first_norm / initial_norm
+    elif current_norm >= target * initial_norm:
+        P = M(best_p_current)
+        return max_iter, P.reshape(nx, ny, nz, 3), lowest_norm / initial_norm, first_norm / initial_norm
+    P = M(p_current)
+    return i + 1, P.reshape(nx, ny, nz, 3), lowest_norm / initial_norm, first_norm / initial_norm
+
+
+def prepare_interaction_matrices(grid, k, d, alpha_array, refract_mult=1.0, expansion=None):
+    """inv_alpha, mask, reduced Ghat, preconditioner (ddaforgit.py:1086-1159)."""
+    a = np.asarray(alpha_array, dtype=np.complex128)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        inv_alpha = np.where(a != 0, 1.0 / np.where(a != 0, a, 1.0), 0)
+    mask = a != 0
+    G = generate_interaction_row(*grid, k, d)
+    M, _ = build_preconditioner(grid, k, d, inv_alpha, refract_mult, expansion)
+    return inv_alpha, mask, G, M
+
+
+def run_solver(grid_size, k, alpha_array, lattice_spacing, refract_mult=1.0, E_inc=None,
+               precon=True, max_iter=20000, target=1e-5, ratio=100, expansion=None):
+    """run_solver with ratio forced to 100 (ddaforgit.py:1175-1210), complex128."""
+    ia, mask, G, M = prepare_interaction_matrices(grid_size, k, lattice_spacing, alpha_array,
+                                                  refract_mult, expansion)
+    return solve_gpbicgstab(grid_size, ia, G, M, E_inc, mask, ratio, max_iter, target, precon)
+
+
+def true_residual(P, E_inc, inv_alpha, mask, G, grid):
+    """||b - A P|| / ||b|| in complex128, b = E_inc * mask."""
+    b = (np.asarray(E_inc) * np.asarray(mask)[..., None]).ravel()
+    ia = np.repeat(np.asarray(inv_alpha).ravel(), 3)
+    r = b - interaction_mult(np.asarray(P).ravel(), G, ia, mask, grid)
+    return float(np.linalg.norm(r) / np.linalg.norm(b))
+
+
+# ----------------------------------------------------------------------------------
+# Mixed precision: complex64 inner solves, FP64 true-residual refinement
+# ----------------------------------------------------------------------------------
+def solve_mixed(grid, inv_alpha, G64, M64, E_inc, mask, tol=1e-6, inner_target=1e-3,
+                max_outer=20, max_iter=2000, ratio=100, precon=True):
+    """Defect correction: x += solve_c64(b - A64 x), stop at FP64 ||b - A x||/||b|| < tol."""
+    nx, ny, nz = grid
+    b = (np.asarray(E_inc) * np.asarray(mask)[..., None]).astype(np.complex128)
+    ia = np.repeat(np.asarray(inv_alpha).ravel(), 3)
+    bn = float(np.linalg.norm(b))
+    x = np.zeros(b.size, dtype=np.complex128)
+    r = b.ravel().copy()
+    total = 0
+    for outer in range(max_outer):
+        rn = float(np.linalg.norm(r))
+        if rn / bn < tol:
+            return x.reshape(nx, ny, nz, 3), total, outer, rn / bn
+        it, d, _, _ = solve_gpbicgstab(grid, inv_alpha, G64, M64, (r / rn).reshape(nx, ny, nz, 3),
+                                       mask, ratio, max_iter, inner_target, precon, dtype=np.complex64)
+        total += abs(it)
+        x = x + rn * d.ravel().astype(np.complex128)
+        r = b.ravel() - interaction_mult(x, G64, ia, mask, grid)
+    rn = float(np.linalg.norm(r))
+    return x.reshape(nx, ny, nz, 3), total, max_outer, rn / bn
+
+
+# ----------------------------------------------------------------------------------
+# Cross sections (absent in the reference; Draine & Flatau 1994, JOSA A 11:1491)
+# ----------------------------------------------------------------------------------
+def cross_sections(P, E_inc, inv_alpha, k, E0=1.0):
+    """C_ext = 4 pi k/|E0|^2 sum Im(E_inc* . P);
+    C_abs = 4 pi k/|E0|^2 sum [Im(P . (inv_alpha)* P*) - (2/3) k^3 |P|^2]."""
+    P = np.asarray(P).reshape(-1, 3)
+    E = np.asarray(E_inc).reshape(-1, 3)
+    ia = np.asarray(inv_alpha).reshape(-1)
+    occ = ia != 0
+    pre = 4.0 * np.pi * k / (abs(E0) ** 2)
+    cext = pre * float(np.sum(np.imag(np.conj(E[occ]) * P[occ])))
+    p2 = np.sum(np.abs(P[occ]) ** 2, axis=1)
+    cabs = pre * float(np.sum(-np.imag(ia[occ]) * p2 - (2.0 / 3.0) * k ** 3 * p2))
+    return cext, cabs
+
+
+def c_sca_farfield(P, k, d=1.0, n_theta=96, n_phi=96):
+    """C_sca = k^4/|E0|^2 int |sum_j (P_j - n(n.P_j)) e^{-ik n.r_j}|^2 dOmega.
+
+    Independent far-field quadrature (Gauss-Legendre in cos(theta), uniform in phi),
+    used to pin cross_sections through C_ext = C_abs + C_sca.
+    """
+    P = np.asarray(P)
+    nx, ny, nz = P.shape[:3]
+    idx = np.argwhere(np.ones((nx, ny, nz), dtype=bool)).astype(np.float64) * d
+    Pf = P.reshape(-1, 3)
+    keep = np.any(Pf != 0, axis=1)
+    idx = idx[keep]
+    Pf = Pf[keep]
+    mu, wmu = np.polynomial.legendre.leggauss(n_theta)
+    phi = np.arange(n_phi) * 2 * np.pi / n_phi
+    total = 0.0
+    for ct, wt in zip(mu, wmu):
+        st = math.sqrt(max(0.0, 1 - ct * ct))
+        n = np.stack([st * np.cos(phi), st * np.sin(phi), np.full_like(phi, ct)], axis=1)  # (nphi,3)
+        ph = np.exp(-1j * k * (idx @ n.T))  # (M, nphi)
+        F = Pf.T @ ph  # (3, nphi)
+        nF = np.sum(n.T * F, axis=0)
+        Ft = F - n.T * nF
+        total += wt * (2 * np.pi / n_phi) * float(np.sum(np.abs(Ft) ** 2))
+    return k ** 4 * total
+
+
+# ----------------------------------------------------------------------------------
+# Dense DDA matrix (pins the FFT matvec and the solver)
+# ----------------------------------------------------------------------------------
+def dense_matrix(grid, k, d, inv_alpha, occupied_only=False):
+    """Explicit matrix of the operator x -> inv_alpha x - mask conv(x) on the box.
+
+    Built from direct pair sums of the dipole field in the standard form
+    G(r) p = e^{ikr}/r^3 {k^2 r^2 (r x p) x r / r^2 + (1 - ikr)[3 r(r.p)/r^2 - p]}: the
+    entry for rows j (occupied) and columns l != j is -G(r_j - r_l), the diagonal block
+    is inv_alpha_j I. Rows of empty sites are zero (masked output). With
+    ``occupied_only`` the columns are restricted to occupied sites too (the system the
+    solver's masked iterates live in); otherwise all N sites are columns, which matches
+    the FFT matvec applied to an arbitrary box vector.
+    Returns (matrix, row_sites, col_sites).
+    """
+    nx, ny, nz = grid
+    N = nx * ny * nz
+    pos = np.argwhere(np.ones(grid, dtype=bool)).astype(np.float64) * d
+    ia = np.asarray(inv_alpha).reshape(-1)
+    occ = np.nonzero(ia != 0)[0]
+    rows = occ if occupied_only else np.arange(N)
+    cols = occ if occupied_only else np.arange(N)
+    A = np.zeros((len(rows), 3, len(cols), 3), dtype=np.complex128)
+    I3 = np.eye(3)
+    colpos = pos[cols]
+    col_index = {int(c): i for i, c in enumerate(cols)}
+    for ri, j in enumerate(rows):
+        if ia[j] == 0:
+            continue
+        R = pos[j] - colpos
+        r = np.sqrt((R * R).sum(axis=1))
+        self_i = col_index.get(int(j))
+        if self_i is not None:
+            r[self_i] = 1.0
+        n = R / r[:, None]
+        nn = n[:, :, None] * n[:, None, :]
+        pre = np.exp(1j * k * r) / r
+        Gt = pre[:, None, None] * (k * k * (I3 - nn) + ((1 - 1j * k * r) / (r * r))[:, None, None] * (3 * nn - I3))
+        if self_i is not None:
+            Gt[self_i] = 0.0
+        A[ri] = -np.transpose(Gt, (1, 0, 2))
+        if self_i is not None:
+            A[ri, :, self_i, :] += ia[j] * I3
+    return A.reshape(3 * len(rows), 3 * len(cols)), rows, cols
+
+
+# ----------------------------------------------------------------------------------
+# Benchmark targets (sizes and rules of the five workloads)
+# ----------------------------------------------------------------------------------
+def a_eff(n_dipoles, d=1.0):
+    return (3.0 * n_dipoles / (4.0 * np.pi)) ** (1.0 / 3.0) * d
+
+
+def sphere_problem(R, m, x, kdir=(0, 0, 1), epol=(1, 0, 0), d=1.0):
+    """Sphere of radius R d centred at (R,R,R) d: the 2R cubic box after trimming."""
+    shapes = [{"type": "sphere", "center": [R * d] * 3, "radius": R * d, "n": m}]
+    g0 = trim_zero_faces(mark_shapes(grid_extents(snap_centers(shapes, d), d), d, snap_centers(shapes, d), [1.0]))
+    ndip = int(np.count_nonzero(g0))
+    k = x / a_eff(ndip, d)
+    return build_problem(shapes, kdir, epol, 2 * np.pi / k, d), ndip

@@ -1,0 +1,110 @@
+"""ctypes binding of libpdda.so (the C ABI declared in include/pdda.h).
+
+The library is required: every operator raises if it is missing or fails to load.
+There is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import torch
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libpdda.so")
+
+PDDA_MODE_FIELD = 0
+PDDA_MODE_PRECOND = 1
+PDDA_MODE_INTERACT = 2
+PDDA_MODE_RESIDUAL = 3
+
+ST_IDLE, ST_RUNNING, ST_CONVERGED, ST_FAILED, ST_MAXITER = range(5)
+
+_c = ctypes
+_vp = _c.c_void_p
+_i = _c.c_int
+_ll = _c.c_longlong
+_d = _c.c_double
+
+
+class SlotStatus(_c.Structure):
+    _fields_ = [("status", _i), ("iters", _i), ("resets", _i), ("flags", _i),
+                ("initial_norm", _d), ("current_norm", _d), ("lowest_norm", _d), ("first_norm", _d)]
+
+
+class MixedReport(_c.Structure):
+    _fields_ = [("outer", _i), ("inner_iters", _i), ("final_rel", _d), ("reserved", _d)]
+
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "pdda_abi_version": (_i, []),
+    "pdda_last_error": (_c.c_char_p, []),
+    "pdda_fft_length_supported": (_i, [_i]),
+    "pdda_green_build": (_i, [_i, _i, _i, _d, _d, _vp, _vp, _vp]),
+    "pdda_precond_build": (_i, [_i, _i, _i, _d, _d, _vp, _vp, _vp, _vp]),
+    "pdda_conv_create": (_i, [_c.POINTER(_vp), _i, _i, _i, _i, _i, _i, _i, _i]),
+    "pdda_conv_destroy": (_i, [_vp]),
+    "pdda_conv_workspace_bytes": (_ll, [_vp]),
+    "pdda_conv_apply": (_i, [_vp, _i, _vp, _vp, _vp, _ll, _vp, _vp, _ll, _vp, _vp]),
+    "pdda_dot": (_i, [_i, _vp, _vp, _ll, _c.POINTER(_d), _vp]),
+    "pdda_zeta_eta": (_i, [_i, _vp, _vp, _vp, _ll, _c.POINTER(_d), _vp]),
+    "pdda_cross_sections": (_i, [_i, _vp, _vp, _vp, _ll, _d, _c.POINTER(_d), _vp]),
+    "pdda_solver_create": (_i, [_c.POINTER(_vp), _i, _i, _i, _i, _i, _i, _i, _i, _i]),
+    "pdda_solver_destroy": (_i, [_vp]),
+    "pdda_solver_bind": (_i, [_vp, _vp, _ll, _vp, _ll, _vp, _vp, _ll]),
+    "pdda_solver_load": (_i, [_vp, _i, _vp, _d, _i, _d, _i, _vp]),
+    "pdda_solver_iterate": (_i, [_vp, _i, _vp]),
+    "pdda_solver_status": (_i, [_vp, _c.POINTER(SlotStatus), _i, _vp]),
+    "pdda_solver_result": (_i, [_vp, _vp, _vp]),
+    "pdda_mixed_create": (_i, [_c.POINTER(_vp), _i, _i, _i, _i, _i, _i, _i, _i]),
+    "pdda_mixed_destroy": (_i, [_vp]),
+    "pdda_mixed_bind": (_i, [_vp, _vp, _vp, _ll, _vp, _ll, _vp, _vp, _vp, _ll]),
+    "pdda_mixed_solve": (_i, [_vp, _vp, _d, _d, _i, _i, _d, _i, _i, _vp]),
+    "pdda_mixed_result": (_i, [_vp, _vp, _c.POINTER(MixedReport), _i, _vp]),
+}
+
+EXPORTED_SYMBOLS = tuple(_SIGS.keys())
+
+_LIB = None
+
+
+class PddaError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"libpdda error {code}: {msg}")
+        self.code = code
+
+
+def load_library(path: str = LIB_PATH):
+    """Loads libpdda.so and declares every entry point; raises if it is absent."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with `python -m precondda_amd.build`")
+    lib = _c.CDLL(path)
+    for name, (res, args) in _SIGS.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _LIB = lib
+    return lib
+
+
+def lib():
+    return load_library()
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        raise PddaError(rc, lib().pdda_last_error().decode(errors="replace"))
+
+
+def stream_ptr(stream=None) -> int:
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return int(s.cuda_stream)
+
+
+def require_gpu() -> None:
+    if not torch.cuda.is_available():
+        raise RuntimeError("precondda_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+    load_library()

@@ -1,0 +1,360 @@
+// extern "C" boundary of libpdda.so (declared in include/pdda.h).
+#include "../../include/pdda.h"
+
+#include <new>
+
+#include "krylov.h"
+
+using namespace pdda;
+
+struct pdda_conv {
+  ConvPlanBase p;
+};
+struct pdda_solver {
+  int prec;
+  SolverT<float> s32;
+  SolverT<double> s64;
+};
+struct pdda_mixed {
+  MixedSolver m;
+};
+
+namespace {
+
+constexpr int RNT = 256;
+constexpr int RNB = 256;  // fixed grid of the standalone reductions (deterministic order)
+
+// Standalone reductions: kind 0 = <a,b>; 1 = the six zeta/eta sums of (y,t,s);
+// 2 = cross-section sums over sites (a = P, b = E, c = inv_alpha).
+template <typename T, int KIND>
+__global__ __launch_bounds__(RNT) void k_red_partial(const C<T>* __restrict__ a, const C<T>* __restrict__ b,
+                                                     const C<T>* __restrict__ c, long long n, double* partial) {
+  constexpr int ND = KIND == 1 ? 6 : (KIND == 2 ? 2 : 1);
+  __shared__ double scratch[2 * ND * (RNT / 64)];
+  double re[ND], im[ND];
+  for (int d = 0; d < ND; ++d) { re[d] = 0.0; im[d] = 0.0; }
+  for (long long i = blockIdx.x * (long long)RNT + threadIdx.x; i < n; i += (long long)RNB * RNT) {
+    if constexpr (KIND == 0) {
+      const C<T> x = a[i], y = b[i];
+      re[0] += (double)x.x * y.x + (double)x.y * y.y;
+      im[0] += (double)x.x * y.y - (double)x.y * y.x;
+    } else if constexpr (KIND == 1) {
+      const C<T> yv = a[i], tv = b[i], sv = c[i];
+      const double yx = yv.x, yy = yv.y, tx = tv.x, ty = tv.y, sx = sv.x, sy = sv.y;
+      re[0] += yx * yx + yy * yy;
+      re[1] += tx * tx + ty * ty;
+      re[2] += yx * tx + yy * ty; im[2] += yx * ty - yy * tx;
+      re[3] += tx * yx + ty * yy; im[3] += tx * yy - ty * yx;
+      re[4] += yx * sx + yy * sy; im[4] += yx * sy - yy * sx;
+      re[5] += tx * sx + ty * sy; im[5] += tx * sy - ty * sx;
+    } else {
+      // site i: P[3i..3i+2], E[3i..], inv_alpha[i]
+      const C<T> ia = c[i];
+      if (ia.x != T(0) || ia.y != T(0)) {
+        double p2 = 0.0, ext = 0.0;
+        for (int q = 0; q < 3; ++q) {
+          const C<T> pv = a[3 * i + q], ev = b[3 * i + q];
+          ext += (double)ev.x * pv.y - (double)ev.y * pv.x;  // Im(conj(E) P)
+          p2 += (double)pv.x * pv.x + (double)pv.y * pv.y;
+        }
+        re[0] += ext;
+        re[1] += -(double)ia.y * p2;
+        im[1] += p2;  // |P|^2 total, for the radiative-reaction term
+      }
+    }
+  }
+  block_sum<ND>(re, im, scratch);
+  if (threadIdx.x == 0)
+    for (int d = 0; d < ND; ++d) {
+      partial[(blockIdx.x * ND + d) * 2] = re[d];
+      partial[(blockIdx.x * ND + d) * 2 + 1] = im[d];
+    }
+}
+
+template <int ND>
+__global__ __launch_bounds__(RNT) void k_red_final(const double* partial, double* out) {
+  __shared__ double scratch[2 * ND * (RNT / 64)];
+  double re[ND], im[ND];
+  for (int d = 0; d < ND; ++d) {
+    re[d] = partial[(threadIdx.x * ND + d) * 2];
+    im[d] = partial[(threadIdx.x * ND + d) * 2 + 1];
+  }
+  block_sum<ND>(re, im, scratch);
+  if (threadIdx.x == 0)
+    for (int d = 0; d < ND; ++d) { out[2 * d] = re[d]; out[2 * d + 1] = im[d]; }
+}
+
+template <typename T, int KIND>
+int run_reduction(const void* a, const void* b, const void* c, long long n, double* host_out, hipStream_t s) {
+  constexpr int ND = KIND == 1 ? 6 : (KIND == 2 ? 2 : 1);
+  double* buf = nullptr;
+  PDDA_HIP_CHECK(hipMalloc(&buf, (RNB * ND * 2 + ND * 2) * sizeof(double)));
+  hipLaunchKernelGGL((k_red_partial<T, KIND>), dim3(RNB), dim3(RNT), 0, s, (const C<T>*)a, (const C<T>*)b,
+                     (const C<T>*)c, n, buf);
+  hipLaunchKernelGGL(k_red_final<ND>, dim3(1), dim3(RNT), 0, s, (const double*)buf, buf + RNB * ND * 2);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(host_out, buf + RNB * ND * 2, ND * 2 * sizeof(double), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(buf);
+  if (e != hipSuccess) {
+    set_error(std::string("reduction: ") + hipGetErrorString(e));
+    return PDDA_ERR_HIP;
+  }
+  return PDDA_OK;
+}
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    return f();
+  } catch (const std::bad_alloc&) {
+    set_error("host out of memory");
+    return PDDA_ERR_OOM;
+  } catch (...) {
+    set_error("unexpected exception");
+    return PDDA_ERR_STATE;
+  }
+}
+
+template <typename T>
+int bind_t(SolverT<T>& s, const void* g, long long gs, const void* m, long long ms, const void* ia,
+                  const void* mask, long long ss) {
+  if (!g || !ia || !mask || (s.precon && !m)) { set_error("solver_bind: null operand"); return PDDA_ERR_ARG; }
+  s.ghat = (const C<T>*)g;
+  s.ghat_stride = gs;
+  s.mhat = (const C<T>*)m;
+  s.mhat_stride = ms;
+  s.inv_alpha = (const C<T>*)ia;
+  s.mask = (const uint8_t*)mask;
+  s.site_stride = ss;
+  return PDDA_OK;
+}
+
+template <typename T>
+void fill_status(const SolverT<T>& s, pdda_slot_status* out, int n) {
+  for (int q = 0; q < n && q < s.nslots; ++q) {
+    const SlotCtl& c = s.ctl_host[q];
+    out[q].status = c.status;
+    out[q].iters = c.status == ST_MAXITER ? c.max_iter : c.stop_it + 1;
+    if (c.status == ST_RUNNING) out[q].iters = c.it;
+    out[q].resets = c.resets;
+    out[q].flags = c.flags;
+    out[q].initial_norm = c.initial_norm;
+    out[q].current_norm = c.current_norm;
+    out[q].lowest_norm = c.lowest_norm;
+    out[q].first_norm = c.first_norm;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int pdda_abi_version(void) { return PDDA_ABI_VERSION; }
+const char* pdda_last_error(void) { return last_error(); }
+int pdda_fft_length_supported(int L) { return fft_length_supported(L) ? 1 : 0; }
+
+int pdda_green_build(int nx, int ny, int nz, double k, double d, void* out_c128, void* out_c64, void* stream) {
+  return guarded([&] { return green_build(nx, ny, nz, k, d, out_c128, out_c64, (hipStream_t)stream); });
+}
+
+int pdda_precond_build(int hx, int hy, int hz, double gre, double gim, const void* g, void* out128, void* out64,
+                       void* stream) {
+  return guarded([&] { return precond_build(hx, hy, hz, gre, gim, g, out128, out64, (hipStream_t)stream); });
+}
+
+int pdda_conv_create(pdda_conv** out, int prec, int nx, int ny, int nz, int hx, int hy, int hz, int nslots) {
+  return guarded([&] {
+    if (!out) { set_error("null output handle"); return PDDA_ERR_ARG; }
+    auto* c = new pdda_conv();
+    int rc = conv_plan_init(&c->p, prec, nx, ny, nz, hx, hy, hz, nslots);
+    if (rc) { conv_plan_free(&c->p); delete c; return rc; }
+    *out = c;
+    return PDDA_OK;
+  });
+}
+
+int pdda_conv_destroy(pdda_conv* c) {
+  if (!c) return PDDA_OK;
+  conv_plan_free(&c->p);
+  delete c;
+  return PDDA_OK;
+}
+
+long long pdda_conv_workspace_bytes(const pdda_conv* c) {
+  if (!c) return 0;
+  const size_t es = c->p.prec ? sizeof(double2) : sizeof(float2);
+  return (long long)((c->p.g.a_stride + c->p.g.b_stride) * c->p.g.nslots * es);
+}
+
+int pdda_conv_apply(pdda_conv* c, int mode, const void* x, void* out, const void* table, long long table_stride,
+                    const void* inv_alpha, const void* mask, long long site_stride, const void* b, void* stream) {
+  if (!c) { set_error("null plan"); return PDDA_ERR_ARG; }
+  ConvArgs a;
+  a.mode = mode;
+  a.x = x;
+  a.out = out;
+  a.table = table;
+  a.table_stride = table_stride;
+  a.inv_alpha = inv_alpha;
+  a.mask = (const uint8_t*)mask;
+  a.site_stride = site_stride;
+  a.b = b;
+  return guarded([&] { return conv_apply(&c->p, a, (hipStream_t)stream); });
+}
+
+int pdda_dot(int prec, const void* a, const void* b, long long n, double* out2, void* stream) {
+  if (!a || !b || !out2 || n < 0) { set_error("pdda_dot: bad arguments"); return PDDA_ERR_ARG; }
+  return prec ? run_reduction<double, 0>(a, b, nullptr, n, out2, (hipStream_t)stream)
+              : run_reduction<float, 0>(a, b, nullptr, n, out2, (hipStream_t)stream);
+}
+
+int pdda_zeta_eta(int prec, const void* y, const void* t, const void* s, long long n, double* out4, void* stream) {
+  if (!y || !t || !s || !out4 || n < 0) { set_error("pdda_zeta_eta: bad arguments"); return PDDA_ERR_ARG; }
+  double sums[12];
+  int rc = prec ? run_reduction<double, 1>(y, t, s, n, sums, (hipStream_t)stream)
+                : run_reduction<float, 1>(y, t, s, n, sums, (hipStream_t)stream);
+  if (rc) return rc;
+  const cd yy{sums[0], sums[1]}, tt{sums[2], sums[3]}, yt{sums[4], sums[5]};
+  const cd ty{sums[6], sums[7]}, ys{sums[8], sums[9]}, ts{sums[10], sums[11]};
+  const cd den = cd_sub(cd_mul(tt, yy), cd_mul(yt, ty));
+  const cd zeta = cd_div(cd_sub(cd_mul(yy, ts), cd_mul(ys, ty)), den);
+  const cd eta = cd_div(cd_sub(cd_mul(tt, ys), cd_mul(yt, ts)), den);
+  out4[0] = zeta.x; out4[1] = zeta.y; out4[2] = eta.x; out4[3] = eta.y;
+  return PDDA_OK;
+}
+
+int pdda_cross_sections(int prec, const void* P, const void* E, const void* ia, long long nsites, double k,
+                        double* out2, void* stream) {
+  if (!P || !E || !ia || !out2 || nsites < 0) { set_error("pdda_cross_sections: bad arguments"); return PDDA_ERR_ARG; }
+  double sums[4];
+  int rc = prec ? run_reduction<double, 2>(P, E, ia, nsites, sums, (hipStream_t)stream)
+                : run_reduction<float, 2>(P, E, ia, nsites, sums, (hipStream_t)stream);
+  if (rc) return rc;
+  const double pre = 4.0 * 3.14159265358979323846 * k;
+  out2[0] = pre * sums[0];
+  out2[1] = pre * (sums[2] - (2.0 / 3.0) * k * k * k * sums[3]);
+  return PDDA_OK;
+}
+
+int pdda_solver_create(pdda_solver** out, int prec, int nx, int ny, int nz, int px, int py, int pz, int nslots,
+                       int precon) {
+  return guarded([&] {
+    if (!out) { set_error("null output handle"); return PDDA_ERR_ARG; }
+    if (prec != 0 && prec != 1) { set_error("prec must be 0 or 1"); return PDDA_ERR_ARG; }
+    auto* s = new pdda_solver();
+    s->prec = prec;
+    int rc = prec ? s->s64.init(nx, ny, nz, px, py, pz, nslots, precon)
+                  : s->s32.init(nx, ny, nz, px, py, pz, nslots, precon);
+    if (rc) {
+      if (prec) s->s64.release(); else s->s32.release();
+      delete s;
+      return rc;
+    }
+    *out = s;
+    return PDDA_OK;
+  });
+}
+
+int pdda_solver_destroy(pdda_solver* s) {
+  if (!s) return PDDA_OK;
+  if (s->prec) s->s64.release(); else s->s32.release();
+  delete s;
+  return PDDA_OK;
+}
+
+int pdda_solver_bind(pdda_solver* s, const void* ghat, long long gs, const void* mhat, long long ms,
+                     const void* ia, const void* mask, long long ss) {
+  if (!s) { set_error("null solver"); return PDDA_ERR_ARG; }
+  return s->prec ? bind_t(s->s64, ghat, gs, mhat, ms, ia, mask, ss) : bind_t(s->s32, ghat, gs, mhat, ms, ia, mask, ss);
+}
+
+int pdda_solver_load(pdda_solver* s, int slot, const void* E, double target, int max_iter, double ratio,
+                     int stagnation, void* stream) {
+  if (!s || !E) { set_error("solver_load: null argument"); return PDDA_ERR_ARG; }
+  return guarded([&] {
+    return s->prec ? s->s64.load(slot, E, target, max_iter, ratio, stagnation, (hipStream_t)stream)
+                   : s->s32.load(slot, E, target, max_iter, ratio, stagnation, (hipStream_t)stream);
+  });
+}
+
+int pdda_solver_iterate(pdda_solver* s, int n_iter, void* stream) {
+  if (!s || n_iter < 0) { set_error("solver_iterate: bad argument"); return PDDA_ERR_ARG; }
+  return guarded([&] {
+    return s->prec ? s->s64.iterate(n_iter, (hipStream_t)stream) : s->s32.iterate(n_iter, (hipStream_t)stream);
+  });
+}
+
+int pdda_solver_status(pdda_solver* s, pdda_slot_status* out, int n, void* stream) {
+  if (!s || !out) { set_error("solver_status: null argument"); return PDDA_ERR_ARG; }
+  int rc = s->prec ? s->s64.status((hipStream_t)stream) : s->s32.status((hipStream_t)stream);
+  if (rc) return rc;
+  if (s->prec) fill_status(s->s64, out, n); else fill_status(s->s32, out, n);
+  return PDDA_OK;
+}
+
+int pdda_solver_result(pdda_solver* s, void* P, void* stream) {
+  if (!s || !P) { set_error("solver_result: null argument"); return PDDA_ERR_ARG; }
+  return guarded([&] {
+    return s->prec ? s->s64.result(P, (hipStream_t)stream) : s->s32.result(P, (hipStream_t)stream);
+  });
+}
+
+int pdda_mixed_create(pdda_mixed** out, int nx, int ny, int nz, int px, int py, int pz, int nslots, int precon) {
+  return guarded([&] {
+    if (!out) { set_error("null output handle"); return PDDA_ERR_ARG; }
+    auto* m = new pdda_mixed();
+    int rc = m->m.init(nx, ny, nz, px, py, pz, nslots, precon);
+    if (rc) { m->m.release(); delete m; return rc; }
+    *out = m;
+    return PDDA_OK;
+  });
+}
+
+int pdda_mixed_destroy(pdda_mixed* m) {
+  if (!m) return PDDA_OK;
+  m->m.release();
+  delete m;
+  return PDDA_OK;
+}
+
+int pdda_mixed_bind(pdda_mixed* m, const void* g128, const void* g64, long long gs, const void* m64, long long ms,
+                    const void* ia128, const void* ia64, const void* mask, long long ss) {
+  if (!m || !g128 || !ia128) { set_error("mixed_bind: null operand"); return PDDA_ERR_ARG; }
+  int rc = bind_t(m->m.inner, g64, gs, m64, ms, ia64, mask, ss);
+  if (rc) return rc;
+  m->m.ghat64 = (const double2*)g128;
+  m->m.ghat64_stride = gs;
+  m->m.inv_alpha64 = (const double2*)ia128;
+  return PDDA_OK;
+}
+
+int pdda_mixed_solve(pdda_mixed* m, const void* E, double tol, double inner_target, int max_outer, int max_inner,
+                     double ratio, int stagnation, int check_every, void* stream) {
+  if (!m || !E || check_every <= 0) { set_error("mixed_solve: bad argument"); return PDDA_ERR_ARG; }
+  if (!m->m.inner.mask) { set_error("mixed_solve: operands not bound"); return PDDA_ERR_STATE; }
+  return guarded([&] {
+    return m->m.solve(E, tol, inner_target, max_outer, max_inner, ratio, stagnation, check_every,
+                      (hipStream_t)stream);
+  });
+}
+
+int pdda_mixed_result(pdda_mixed* m, void* P, pdda_mixed_report* rep, int n, void* stream) {
+  if (!m) { set_error("mixed_result: null"); return PDDA_ERR_ARG; }
+  if (P) {
+    PDDA_HIP_CHECK(hipMemcpyAsync(P, m->m.x64, (size_t)m->m.n3 * m->m.nslots * sizeof(double2),
+                                  hipMemcpyDeviceToDevice, (hipStream_t)stream));
+    PDDA_HIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
+  }
+  if (rep) {
+    for (int q = 0; q < n && q < m->m.nslots && q < (int)m->m.outer_iters.size(); ++q) {
+      rep[q].outer = m->m.outer_iters[q];
+      rep[q].inner_iters = m->m.inner_iters[q];
+      rep[q].final_rel = m->m.final_rel[q];
+      rep[q].reserved = 0.0;
+    }
+  }
+  return PDDA_OK;
+}
+
+}  // extern "C"

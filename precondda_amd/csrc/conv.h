@@ -1,0 +1,80 @@
+// Pruned 3-D FFT convolution with a reduced, mirror-symmetric 3x3 tensor table:
+// the interaction matvec, the preconditioner apply and the excited-field product.
+//
+// Five passes over interleaved (x, y, z, component) data (L = FFT length per axis,
+// n = box size per axis, table half sizes h = L/2):
+//   P1 z-fwd   box x (nx,ny,nz,3)     -> A (nx,ny,Lz,3)   rows nz..Lz-1 are zeros, never read
+//   P2 y-fwd   A                      -> B (nx,Ly,Lz,3)
+//   P3 x-fwd, 3x3 tensor product, x-inv, in place on B (only the nx kept rows are read/written)
+//   P4 y-inv   B                      -> A (rows < ny kept)
+//   P5 z-inv   A                      -> box (rows < nz kept) with the epilogue
+//               (inv_alpha x - mask conv, mask conv, conv or b - (inv_alpha x - mask conv))
+//               and optional fused dot products for the Krylov scalars.
+#pragma once
+
+#include "fft_tile.h"
+#include "solver_state.h"
+
+namespace pdda {
+
+enum : int {
+  MODE_FIELD = 0,     // out = conv                       (excited field, no mask)
+  MODE_PRECOND = 1,   // out = mask * conv                (preconditioner apply)
+  MODE_INTERACT = 2,  // out = inv_alpha x - mask conv    (interaction matvec)
+  MODE_RESIDUAL = 3,  // out = b - (inv_alpha x - mask conv)
+};
+
+struct ConvGeom {
+  int nx, ny, nz;
+  int Lx, Ly, Lz;
+  int nslots;
+  int rows1;  // box rows (fixed x, consecutive y) per tile in P1/P5
+  int ncol2;  // columns per tile in P2/P4
+  int wz3;    // kz values per tile in P3
+  long long vec_stride;  // complex elements per slot in box vectors (nx*ny*nz*3)
+  long long a_stride;    // per slot in A (nx*ny*Lz*3)
+  long long b_stride;    // per slot in B (nx*Ly*Lz*3)
+};
+
+struct ConvPlanBase {
+  int prec;  // 0 = complex64, 1 = complex128
+  ConvGeom g;
+  FftPlan px, py, pz;
+  void* tw[3] = {nullptr, nullptr, nullptr};  // device root tables (plan precision)
+  void* work_a = nullptr;
+  void* work_b = nullptr;
+  cd* partial = nullptr;   // [nslots][nwg5][6]
+  int* counters = nullptr; // [nslots]
+  int nwg5 = 0;            // P5 workgroups per slot
+  size_t lds[5] = {0, 0, 0, 0, 0};
+};
+
+// Arguments of one convolution launch sequence.
+struct ConvArgs {
+  int mode = MODE_FIELD;
+  const void* x = nullptr;      // box vector(s), plan precision
+  void* out = nullptr;          // box vector(s)
+  const void* table = nullptr;  // reduced (hx+1,hy+1,hz+1,6) table(s)
+  long long table_stride = 0;   // complex elements between slot tables (0: shared)
+  const void* inv_alpha = nullptr;  // per site (nx*ny*nz), plan precision
+  const uint8_t* mask = nullptr;    // per site
+  long long site_stride = 0;        // elements between slot site arrays (0: shared)
+  const void* b = nullptr;          // rhs for MODE_RESIDUAL (box vector)
+  int dots = DOT_NONE;
+  const void* d0 = nullptr;  // rhat (ALPHA, RESID) or y (ZETA)
+  const void* d1 = nullptr;  // s (ZETA)
+  SlotCtl* ctl = nullptr;
+  int need = NEED_ALL;
+};
+
+// Host API (conv.hip).
+bool fft_length_supported(int L);
+int make_fft_plan(int L, FftPlan* out);
+int conv_plan_init(ConvPlanBase* p, int prec, int nx, int ny, int nz, int hx, int hy, int hz, int nslots);
+void conv_plan_free(ConvPlanBase* p);
+int conv_apply(ConvPlanBase* p, const ConvArgs& a, hipStream_t s);
+int green_build(int nx, int ny, int nz, double k, double d, void* out64, void* out32, hipStream_t s);
+int precond_build(int hx, int hy, int hz, double gre, double gim, const void* g64, void* out64, void* out32,
+                  hipStream_t s);
+
+}  // namespace pdda

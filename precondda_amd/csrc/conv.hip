@@ -1,0 +1,730 @@
+// Pruned FFT convolution kernels (P1..P5), the Green's-tensor build and the
+// circulant preconditioner build, all on gfx950.
+#include "conv.h"
+
+#include <algorithm>
+#include <cmath>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace pdda {
+
+// ---------------------------------------------------------------------------------
+// Error state
+// ---------------------------------------------------------------------------------
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+const char* last_error() { return g_err.c_str(); }
+
+// ---------------------------------------------------------------------------------
+// Dot-product fusion helpers
+// ---------------------------------------------------------------------------------
+template <int ND>
+__device__ inline void finalize_dots(SlotCtl& c, int kind, const double* re, const double* im) {
+  for (int d = 0; d < ND && d < 6; ++d) c.sums[d] = cd{re[d], im[d]};
+  if (kind == DOT_ALPHA) {
+    c.alpha = cd_div(c.rho1, cd{re[0], im[0]});
+  } else if (kind == DOT_ZETA) {
+    const cd yy{re[0], im[0]}, tt{re[1], im[1]}, yt{re[2], im[2]};
+    const cd ty{re[3], im[3]}, ys{re[4], im[4]}, ts{re[5], im[5]};
+    const bool zeta_only = (c.it == 0) || (c.flags & F_WAS_RESET);
+    if (zeta_only) {
+      c.zeta = cd_div(ts, tt);
+      c.eta = cd{0.0, 0.0};
+    } else {
+      const cd den = cd_sub(cd_mul(tt, yy), cd_mul(yt, ty));
+      c.zeta = cd_div(cd_sub(cd_mul(yy, ts), cd_mul(ys, ty)), den);
+      c.eta = cd_div(cd_sub(cd_mul(tt, ys), cd_mul(yt, ts)), den);
+    }
+  }
+}
+
+// Workgroup partial -> per-slot partial array; the last workgroup of the slot (ticket)
+// sums all partials in a fixed order and finalizes the slot's scalars.
+template <int ND>
+__device__ inline void grid_reduce(double (&re)[ND], double (&im)[ND], cd* partial, int nwg, int wg,
+                                   int* counter, SlotCtl* ctl, int kind) {
+  __shared__ double scratch[2 * ND * (FFT_NT / 64)];
+  __shared__ int is_last;
+  block_sum<ND>(re, im, scratch);
+  if (threadIdx.x == 0) {
+    for (int d = 0; d < ND; ++d) partial[(size_t)wg * ND + d] = cd{re[d], im[d]};
+    __threadfence();
+    const int t = atomicAdd(counter, 1);
+    is_last = (t == nwg - 1);
+  }
+  __syncthreads();
+  if (!is_last) return;
+  __threadfence();
+  const volatile double* pv = reinterpret_cast<const volatile double*>(partial);
+  double r2[ND], i2[ND];
+  for (int d = 0; d < ND; ++d) { r2[d] = 0.0; i2[d] = 0.0; }
+  for (int w = threadIdx.x; w < nwg; w += blockDim.x) {
+    for (int d = 0; d < ND; ++d) {
+      r2[d] += pv[2 * ((size_t)w * ND + d)];
+      i2[d] += pv[2 * ((size_t)w * ND + d) + 1];
+    }
+  }
+  block_sum<ND>(r2, i2, scratch);
+  if (threadIdx.x == 0) {
+    finalize_dots<ND>(*ctl, kind, r2, i2);
+    atomicExch(counter, 0);
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// P1: z-forward over box rows (fixed x, rows1 consecutive y); zero padding nz..Lz-1.
+// ---------------------------------------------------------------------------------
+template <typename T, bool GEN>
+__global__ __launch_bounds__(FFT_NT) void k_zfwd(const C<T>* __restrict__ x, C<T>* __restrict__ A, ConvGeom g,
+                                                 FftPlan pz, const C<T>* __restrict__ twz, const SlotCtl* ctl,
+                                                 int need) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  C<T>* tw = reinterpret_cast<C<T>*>(smem);
+  C<T>* data = tw + g.Lz;
+  const int tiles_y = (g.ny + g.rows1 - 1) / g.rows1;
+  const int t = blockIdx.x;
+  const int slot = t / (g.nx * tiles_y);
+  const int rem = t - slot * (g.nx * tiles_y);
+  const int ix = rem / tiles_y, ty = rem - ix * tiles_y;
+  if (!slot_wanted(ctl, slot, need)) return;
+  const int y0 = ty * g.rows1, rows = min(g.rows1, g.ny - y0), ncol = 3 * rows;
+  load_twiddles<T>(tw, twz, g.Lz);
+  const C<T>* src = x + slot * g.vec_stride + ((long long)ix * g.ny + y0) * g.nz * 3;
+  const int rl = 3 * g.nz, nin = rows * rl;
+  for (int e = threadIdx.x; e < nin; e += FFT_NT) {
+    const int row = e / rl, r2 = e - row * rl, p = r2 / 3, c = r2 - 3 * p;
+    data[p * ncol + row * 3 + c] = src[e];
+  }
+  const int nzero = (g.Lz - g.nz) * ncol;
+  for (int e = threadIdx.x; e < nzero; e += FFT_NT) data[g.nz * ncol + e] = mk<T>(0, 0);
+  __syncthreads();
+  tile_fft<T, false, GEN>(data, ncol, pz, tw);
+  C<T>* dst = A + slot * g.a_stride + ((long long)ix * g.ny + y0) * g.Lz * 3;
+  const int ro = 3 * g.Lz, nout = rows * ro;
+  for (int e = threadIdx.x; e < nout; e += FFT_NT) {
+    const int row = e / ro, r2 = e - row * ro, p = r2 / 3, c = r2 - 3 * p;
+    dst[e] = data[p * ncol + row * 3 + c];
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// Generic strided-axis pass: in viewed as (outer, in_rows, W), transform along rows.
+// Loads rows < n_load (zero to L), stores rows < n_store into out (outer, out_rows, W).
+// Used for P2, P4 and the y/x passes of the Green's-tensor build.
+// ---------------------------------------------------------------------------------
+struct AxisArgs {
+  int n_outer, outer_per_slot;
+  int in_rows, n_load, out_rows, n_store;
+  long long W;
+  int ncol;
+  int negate;
+};
+
+template <typename T, bool INV, bool GEN>
+__global__ __launch_bounds__(FFT_NT) void k_axis(const C<T>* __restrict__ in, C<T>* __restrict__ out, AxisArgs a,
+                                                 FftPlan pl, const C<T>* __restrict__ twl, const SlotCtl* ctl,
+                                                 int need) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  C<T>* tw = reinterpret_cast<C<T>*>(smem);
+  C<T>* data = tw + pl.L;
+  const long long ncb = (a.W + a.ncol - 1) / a.ncol;
+  const long long t = blockIdx.x;
+  const int o = (int)(t / ncb);
+  const long long c0 = (t - (long long)o * ncb) * a.ncol;
+  if (!slot_wanted(ctl, o / a.outer_per_slot, need)) return;
+  const int ncol = (int)min((long long)a.ncol, a.W - c0);
+  load_twiddles<T>(tw, twl, pl.L);
+  const C<T>* src = in + (long long)o * a.in_rows * a.W + c0;
+  const int nl = a.n_load * ncol;
+  for (int e = threadIdx.x; e < nl; e += FFT_NT) {
+    const int r = e / ncol, c = e - r * ncol;
+    data[e] = src[(long long)r * a.W + c];
+  }
+  const int nz0 = (pl.L - a.n_load) * ncol;
+  for (int e = threadIdx.x; e < nz0; e += FFT_NT) data[nl + e] = mk<T>(0, 0);
+  __syncthreads();
+  tile_fft<T, INV, GEN>(data, ncol, pl, tw);
+  C<T>* dst = out + (long long)o * a.out_rows * a.W + c0;
+  const int ns = a.n_store * ncol;
+  for (int e = threadIdx.x; e < ns; e += FFT_NT) {
+    const int r = e / ncol, c = e - r * ncol;
+    C<T> v = data[e];
+    if (a.negate) { v.x = -v.x; v.y = -v.y; }
+    dst[(long long)r * a.W + c] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// P3: x-forward, symmetric 3x3 product with the reduced table, x-inverse, in place.
+// Tile = fixed ky, wz3 consecutive kz (all 3 components), all kx.
+// ---------------------------------------------------------------------------------
+template <typename T, bool GEN>
+__global__ __launch_bounds__(FFT_NT) void k_xconv(C<T>* __restrict__ B, const C<T>* __restrict__ table,
+                                                  long long table_stride, ConvGeom g, FftPlan px,
+                                                  const C<T>* __restrict__ twx, const SlotCtl* ctl, int need,
+                                                  T scale) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  C<T>* tw = reinterpret_cast<C<T>*>(smem);
+  C<T>* data = tw + g.Lx;
+  const int nkzb = (g.Lz + g.wz3 - 1) / g.wz3;
+  const int t = blockIdx.x;
+  const int slot = t / (g.Ly * nkzb);
+  const int rem = t - slot * (g.Ly * nkzb);
+  const int ky = rem / nkzb, kb = rem - ky * nkzb;
+  if (!slot_wanted(ctl, slot, need)) return;
+  const int kz0 = kb * g.wz3, wz = min(g.wz3, g.Lz - kz0), ncol = 3 * wz;
+  const long long W = 3LL * g.Lz;
+  const long long xs = (long long)g.Ly * W;
+  C<T>* base = B + slot * g.b_stride + (long long)ky * W + 3 * kz0;
+  load_twiddles<T>(tw, twx, g.Lx);
+  const int nl = g.nx * ncol;
+  for (int e = threadIdx.x; e < nl; e += FFT_NT) {
+    const int xr = e / ncol, c = e - xr * ncol;
+    data[e] = base[xr * xs + c];
+  }
+  const int nz0 = (g.Lx - g.nx) * ncol;
+  for (int e = threadIdx.x; e < nz0; e += FFT_NT) data[nl + e] = mk<T>(0, 0);
+  __syncthreads();
+  tile_fft<T, false, GEN>(data, ncol, px, tw);
+
+  const int hx = g.Lx / 2, hy = g.Ly / 2, hz = g.Lz / 2;
+  const bool fy = ky > hy;
+  const int my = fy ? g.Ly - ky : ky;
+  const C<T>* tab = table + slot * table_stride;
+  const int ntrip = g.Lx * wz;
+  for (int e = threadIdx.x; e < ntrip; e += FFT_NT) {
+    const int kx = e / wz, q = e - kx * wz, kz = kz0 + q;
+    const bool fx = kx > hx, fz = kz > hz;
+    const int mx = fx ? g.Lx - kx : kx, mz = fz ? g.Lz - kz : kz;
+    const C<T>* gp = tab + (((long long)mx * (hy + 1) + my) * (hz + 1) + mz) * 6;
+    const C<T> g0 = gp[0], g3 = gp[3], g5 = gp[5];
+    C<T> g1 = gp[1], g2 = gp[2], g4 = gp[4];
+    if (fx != fy) g1 = cscale(g1, T(-1));
+    if (fx != fz) g2 = cscale(g2, T(-1));
+    if (fy != fz) g4 = cscale(g4, T(-1));
+    C<T>* dp = data + kx * ncol + 3 * q;
+    const C<T> x0 = dp[0], x1 = dp[1], x2 = dp[2];
+    const C<T> y0 = cadd(cadd(cmul(g0, x0), cmul(g1, x1)), cmul(g2, x2));
+    const C<T> y1 = cadd(cadd(cmul(g1, x0), cmul(g3, x1)), cmul(g4, x2));
+    const C<T> y2 = cadd(cadd(cmul(g2, x0), cmul(g4, x1)), cmul(g5, x2));
+    dp[0] = cscale(y0, scale);
+    dp[1] = cscale(y1, scale);
+    dp[2] = cscale(y2, scale);
+  }
+  __syncthreads();
+  tile_fft<T, true, GEN>(data, ncol, px, tw);
+  for (int e = threadIdx.x; e < nl; e += FFT_NT) {
+    const int xr = e / ncol, c = e - xr * ncol;
+    base[xr * xs + c] = data[e];
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// P5: z-inverse, crop to nz, epilogue, optional fused dots.
+// ---------------------------------------------------------------------------------
+template <typename T, int MODE, int DOTS, bool GEN>
+__global__ __launch_bounds__(FFT_NT) void k_zinv(const C<T>* __restrict__ A, C<T>* __restrict__ out, ConvGeom g,
+                                                 FftPlan pz, const C<T>* __restrict__ twz,
+                                                 const C<T>* __restrict__ xin, const C<T>* __restrict__ ia,
+                                                 const uint8_t* __restrict__ mask, long long site_stride,
+                                                 const C<T>* __restrict__ bvec, const C<T>* __restrict__ d0,
+                                                 const C<T>* __restrict__ d1, cd* partial, int* counters,
+                                                 SlotCtl* ctl, int need, int nwg) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  C<T>* tw = reinterpret_cast<C<T>*>(smem);
+  C<T>* data = tw + g.Lz;
+  const int tiles_y = (g.ny + g.rows1 - 1) / g.rows1;
+  const int t = blockIdx.x;
+  const int slot = t / (g.nx * tiles_y);
+  const int wg = t - slot * (g.nx * tiles_y);
+  const int ix = wg / tiles_y, ty = wg - ix * tiles_y;
+  if (!slot_wanted(ctl, slot, need)) return;
+  const int y0 = ty * g.rows1, rows = min(g.rows1, g.ny - y0), ncol = 3 * rows;
+  load_twiddles<T>(tw, twz, g.Lz);
+  const C<T>* src = A + slot * g.a_stride + ((long long)ix * g.ny + y0) * g.Lz * 3;
+  const int ri = 3 * g.Lz, nin = rows * ri;
+  for (int e = threadIdx.x; e < nin; e += FFT_NT) {
+    const int row = e / ri, r2 = e - row * ri, p = r2 / 3, c = r2 - 3 * p;
+    data[p * ncol + row * 3 + c] = src[e];
+  }
+  __syncthreads();
+  tile_fft<T, true, GEN>(data, ncol, pz, tw);
+
+  constexpr int ND = DOTS == DOT_ALPHA ? 1 : DOTS == DOT_ZETA ? 6 : DOTS == DOT_RESID ? 2 : 1;
+  double sre[ND], sim[ND];
+#pragma unroll
+  for (int d = 0; d < ND; ++d) { sre[d] = 0.0; sim[d] = 0.0; }
+
+  const long long site0 = ((long long)ix * g.ny + y0) * g.nz;
+  const long long vbase = slot * g.vec_stride + site0 * 3;
+  const long long sbase = slot * site_stride + site0;
+  const int ro = 3 * g.nz, nout = rows * ro;
+  for (int e = threadIdx.x; e < nout; e += FFT_NT) {
+    const int row = e / ro, r2 = e - row * ro, p = r2 / 3, c = r2 - 3 * p;
+    const C<T> cv = data[p * ncol + row * 3 + c];
+    const long long sidx = sbase + (long long)row * g.nz + p;
+    const long long vidx = vbase + e;
+    C<T> o;
+    if (MODE == MODE_FIELD) {
+      o = cv;
+    } else {
+      const T m = mask[sidx] ? T(1) : T(0);
+      const C<T> mc = cscale(cv, m);
+      if (MODE == MODE_PRECOND) {
+        o = mc;
+      } else {
+        const C<T> v = csub(cmul(ia[sidx], xin[vidx]), mc);
+        o = (MODE == MODE_INTERACT) ? v : csub(bvec[vidx], v);
+      }
+    }
+    out[vidx] = o;
+    if (DOTS == DOT_ALPHA) {
+      const C<T> a = d0[vidx];
+      sre[0] += (double)a.x * o.x + (double)a.y * o.y;
+      sim[0] += (double)a.x * o.y - (double)a.y * o.x;
+    } else if (DOTS == DOT_RESID) {
+      const C<T> a = d0[vidx];
+      sre[0] += (double)o.x * o.x + (double)o.y * o.y;
+      sre[1] += (double)a.x * o.x + (double)a.y * o.y;
+      sim[1] += (double)a.x * o.y - (double)a.y * o.x;
+    } else if (DOTS == DOT_ZETA) {
+      const C<T> y = d0[vidx], s = d1[vidx];
+      const double yx = y.x, yy = y.y, tx = o.x, ty2 = o.y, sx = s.x, sy = s.y;
+      sre[0] += yx * yx + yy * yy;                  // <y,y>
+      sre[1] += tx * tx + ty2 * ty2;                // <t,t>
+      sre[2] += yx * tx + yy * ty2; sim[2] += yx * ty2 - yy * tx;  // <y,t>
+      sre[3] += tx * yx + ty2 * yy; sim[3] += tx * yy - ty2 * yx;  // <t,y>
+      sre[4] += yx * sx + yy * sy; sim[4] += yx * sy - yy * sx;    // <y,s>
+      sre[5] += tx * sx + ty2 * sy; sim[5] += tx * sy - ty2 * sx;  // <t,s>
+    }
+  }
+  if (DOTS != DOT_NONE) {
+    grid_reduce<ND>(sre, sim, partial + (size_t)slot * nwg * 6, nwg, wg, counters + slot, ctl + slot, DOTS);
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// Green's tensor: z-pass that evaluates A on the 2n embedding on the fly.
+// Output G1 (2nx, 2ny, nz+1, 6): FFT_z(-A), kz = 0..nz.
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(FFT_NT) void k_green_z(double2* __restrict__ G1, int nx, int ny, int nz, double k,
+                                                    double d, int rows1, FftPlan pz,
+                                                    const double2* __restrict__ twz) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  double2* tw = reinterpret_cast<double2*>(smem);
+  const int Lz = 2 * nz;
+  double2* data = tw + Lz;
+  const int Ly = 2 * ny;
+  const int tiles_y = (Ly + rows1 - 1) / rows1;
+  const int ix = blockIdx.x / tiles_y, ty = blockIdx.x - ix * tiles_y;
+  const int y0 = ty * rows1, rows = min(rows1, Ly - y0), ncol = 6 * rows;
+  load_twiddles<double>(tw, twz, Lz);
+  const double px = (ix >= nx) ? (ix - 2 * nx) * d : ix * d;
+  const int n = rows * Lz;
+  for (int e = threadIdx.x; e < n; e += FFT_NT) {
+    const int row = e / Lz, iz = e - row * Lz, iy = y0 + row;
+    const double py = (iy >= ny) ? (iy - 2 * ny) * d : iy * d;
+    const double pzv = (iz >= nz) ? (iz - 2 * nz) * d : iz * d;
+    double2 val[6];
+    const bool zero = (ix == 0 && iy == 0 && iz == 0) || ix == nx || iy == ny || iz == nz;
+    if (zero) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) val[c] = make_double2(0.0, 0.0);
+    } else {
+      const double r2 = px * px + py * py + pzv * pzv;
+      const double r = sqrt(r2);
+      const double ir = 1.0 / r;
+      const double u[3] = {px * ir, py * ir, pzv * ir};
+      double sn, cs;
+      sincos(k * r, &sn, &cs);
+      // e = exp(ikr)/r ; f2 = (ikr - 1)/r^2
+      const double ex = cs * ir, ey = sn * ir;
+      const double f2x = -1.0 / r2, f2y = k * r / r2;
+      const int ca[6] = {0, 0, 0, 1, 1, 2}, cb[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        const double nc = u[ca[c]] * u[cb[c]];
+        const double dg = (ca[c] == cb[c]) ? 1.0 : 0.0;
+        const double t1 = k * k * (nc - dg);
+        const double t2 = 3.0 * nc - dg;
+        // (t1 + t2 * f2) * e, negated (the table holds FFT(-A))
+        const double bx = t1 + t2 * f2x, by = t2 * f2y;
+        val[c] = make_double2(-(bx * ex - by * ey), -(bx * ey + by * ex));
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) data[iz * ncol + row * 6 + c] = val[c];
+  }
+  __syncthreads();
+  tile_fft<double, false, true>(data, ncol, pz, tw);
+  double2* dst = G1 + ((long long)ix * Ly + y0) * (nz + 1) * 6;
+  const int ro = (nz + 1) * 6, nout = rows * ro;
+  for (int e = threadIdx.x; e < nout; e += FFT_NT) {
+    const int row = e / ro, r2 = e - row * ro, p = r2 / 6, c = r2 - 6 * p;
+    dst[e] = data[p * ncol + row * 6 + c];
+  }
+}
+
+// Cast complex128 -> complex64.
+__global__ void k_cast64to32(const double2* __restrict__ in, float2* __restrict__ out, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    out[i] = make_float2((float)in[i].x, (float)in[i].y);
+}
+
+// Circulant preconditioner: M = (gamma I - G)^-1 per table entry (symmetric 3x3).
+__global__ void k_precond_invert(const double2* __restrict__ G, double2* __restrict__ M, long long n, double gre,
+                                 double gim) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const double2* g = G + i * 6;
+    const cd gam{gre, gim};
+    const cd a = cd_sub(gam, cd{g[0].x, g[0].y}), b = cd{-g[1].x, -g[1].y}, c = cd{-g[2].x, -g[2].y};
+    const cd dd = cd_sub(gam, cd{g[3].x, g[3].y}), e = cd{-g[4].x, -g[4].y};
+    const cd f = cd_sub(gam, cd{g[5].x, g[5].y});
+    const cd A00 = cd_sub(cd_mul(dd, f), cd_mul(e, e));
+    const cd A01 = cd_sub(cd_mul(c, e), cd_mul(b, f));
+    const cd A02 = cd_sub(cd_mul(b, e), cd_mul(c, dd));
+    const cd A11 = cd_sub(cd_mul(a, f), cd_mul(c, c));
+    const cd A12 = cd_sub(cd_mul(b, c), cd_mul(a, e));
+    const cd A22 = cd_sub(cd_mul(a, dd), cd_mul(b, b));
+    const cd det = cd_add(cd_add(cd_mul(a, A00), cd_mul(b, A01)), cd_mul(c, A02));
+    const cd one{1.0, 0.0};
+    const cd id = cd_div(one, det);
+    const cd r[6] = {cd_mul(A00, id), cd_mul(A01, id), cd_mul(A02, id),
+                     cd_mul(A11, id), cd_mul(A12, id), cd_mul(A22, id)};
+    double2* m = M + i * 6;
+    for (int q = 0; q < 6; ++q) m[q] = make_double2(r[q].x, r[q].y);
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------------
+static const int kRadices[] = {8, 4, 2, 3, 5, 7, 11, 13};
+
+bool fft_length_supported(int L) {
+  if (L < 2 || L > 1024) return false;
+  int r = L;
+  for (int p : {2, 3, 5, 7, 11, 13})
+    while (r % p == 0) r /= p;
+  return r == 1;
+}
+
+int make_fft_plan(int L, FftPlan* out) {
+  if (!fft_length_supported(L)) {
+    set_error("FFT length " + std::to_string(L) + " is not a product of 2,3,5,7,11,13 in [2,1024]");
+    return PDDA_ERR_SIZE;
+  }
+  std::vector<int> rs;
+  int r = L, e2 = 0;
+  while (r % 2 == 0) { r /= 2; ++e2; }
+  int n8 = e2 / 3, rem = e2 % 3;
+  if (rem == 1 && n8 > 0) { --n8; rs.push_back(4); rs.push_back(4); }
+  else if (rem == 1) rs.push_back(2);
+  else if (rem == 2) rs.push_back(4);
+  for (int i = 0; i < n8; ++i) rs.insert(rs.begin(), 8);
+  for (int p : {3, 5, 7, 11, 13})
+    while (r % p == 0) { r /= p; rs.push_back(p); }
+  if ((int)rs.size() > FFT_MAX_PASSES) { set_error("too many FFT passes"); return PDDA_ERR_SIZE; }
+  out->L = L;
+  out->npass = (int)rs.size();
+  out->gen = 0;
+  for (int i = 0; i < out->npass; ++i) {
+    out->radix[i] = rs[i];
+    if (rs[i] == 7 || rs[i] == 11 || rs[i] == 13) out->gen = 1;
+  }
+  return PDDA_OK;
+}
+
+static int upload_twiddles(int L, int prec, void** dev) {
+  std::vector<double> re(L), im(L);
+  for (int m = 0; m < L; ++m) {
+    long double ang = -2.0L * 3.141592653589793238462643383279502884L * (long double)m / (long double)L;
+    re[m] = (double)cosl(ang);
+    im[m] = (double)sinl(ang);
+  }
+  if (prec == 1) {
+    std::vector<double2> h(L);
+    for (int m = 0; m < L; ++m) h[m] = make_double2(re[m], im[m]);
+    PDDA_HIP_CHECK(hipMalloc(dev, L * sizeof(double2)));
+    PDDA_HIP_CHECK(hipMemcpy(*dev, h.data(), L * sizeof(double2), hipMemcpyHostToDevice));
+  } else {
+    std::vector<float2> h(L);
+    for (int m = 0; m < L; ++m) h[m] = make_float2((float)re[m], (float)im[m]);
+    PDDA_HIP_CHECK(hipMalloc(dev, L * sizeof(float2)));
+    PDDA_HIP_CHECK(hipMemcpy(*dev, h.data(), L * sizeof(float2), hipMemcpyHostToDevice));
+  }
+  return PDDA_OK;
+}
+
+static int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+int conv_plan_init(ConvPlanBase* p, int prec, int nx, int ny, int nz, int hx, int hy, int hz, int nslots) {
+  if (prec != 0 && prec != 1) { set_error("prec must be 0 (complex64) or 1 (complex128)"); return PDDA_ERR_ARG; }
+  if (nx <= 0 || ny <= 0 || nz <= 0 || nslots <= 0) { set_error("box sizes and slot count must be positive"); return PDDA_ERR_ARG; }
+  if (hx < nx || hy < ny || hz < nz) { set_error("table half sizes must be >= box sizes"); return PDDA_ERR_ARG; }
+  p->prec = prec;
+  ConvGeom& g = p->g;
+  g.nx = nx; g.ny = ny; g.nz = nz;
+  g.Lx = 2 * hx; g.Ly = 2 * hy; g.Lz = 2 * hz;
+  g.nslots = nslots;
+  int rc;
+  if ((rc = make_fft_plan(g.Lx, &p->px))) return rc;
+  if ((rc = make_fft_plan(g.Ly, &p->py))) return rc;
+  if ((rc = make_fft_plan(g.Lz, &p->pz))) return rc;
+  const int cap = prec ? tile_cap<double>() : tile_cap<float>();
+  const size_t es = prec ? sizeof(double2) : sizeof(float2);
+  if (3 * g.Lz > cap || 3 * g.Lx > cap || g.Ly > cap) {
+    set_error("FFT length too long for one LDS tile at this precision");
+    return PDDA_ERR_SIZE;
+  }
+  const long long target = 2048;  // workgroups per pass to fill 256 CUs
+  // P1/P5: rows of one x-plane
+  {
+    int rmax = std::min(ny, cap / (3 * g.Lz));
+    long long want = std::max(1LL, (long long)nx * ny * nslots / target);
+    g.rows1 = (int)std::max(1LL, std::min((long long)rmax, want));
+  }
+  // P2/P4: columns of the (Lz,3) axis
+  {
+    const long long W = 3LL * g.Lz;
+    int cmax = (int)std::min<long long>(W, cap / g.Ly);
+    long long want = std::max(16LL, W * nx * nslots / target);
+    g.ncol2 = (int)std::min<long long>(cmax, want);
+  }
+  // P3: kz per tile
+  {
+    int wmax = std::min(g.Lz, cap / (3 * g.Lx));
+    long long want = std::max(1LL, (long long)g.Lz * g.Ly * nslots / target);
+    g.wz3 = (int)std::max(1LL, std::min((long long)wmax, want));
+  }
+  g.vec_stride = 3LL * nx * ny * nz;
+  g.a_stride = 3LL * nx * ny * g.Lz;
+  g.b_stride = 3LL * nx * g.Ly * g.Lz;
+  if ((rc = upload_twiddles(g.Lx, prec, &p->tw[0]))) return rc;
+  if ((rc = upload_twiddles(g.Ly, prec, &p->tw[1]))) return rc;
+  if ((rc = upload_twiddles(g.Lz, prec, &p->tw[2]))) return rc;
+  PDDA_HIP_CHECK(hipMalloc(&p->work_a, (size_t)g.a_stride * nslots * es));
+  PDDA_HIP_CHECK(hipMalloc(&p->work_b, (size_t)g.b_stride * nslots * es));
+  const int tiles_y = ceil_div(ny, g.rows1);
+  p->nwg5 = nx * tiles_y;
+  PDDA_HIP_CHECK(hipMalloc(&p->partial, (size_t)nslots * p->nwg5 * 6 * sizeof(cd)));
+  PDDA_HIP_CHECK(hipMalloc(&p->counters, (size_t)nslots * sizeof(int)));
+  PDDA_HIP_CHECK(hipMemset(p->counters, 0, (size_t)nslots * sizeof(int)));
+  p->lds[0] = (size_t)(g.Lz + g.Lz * 3 * g.rows1) * es;
+  p->lds[1] = (size_t)(g.Ly + g.Ly * g.ncol2) * es;
+  p->lds[2] = (size_t)(g.Lx + g.Lx * 3 * g.wz3) * es;
+  p->lds[3] = p->lds[1];
+  p->lds[4] = p->lds[0];
+  return PDDA_OK;
+}
+
+void conv_plan_free(ConvPlanBase* p) {
+  for (int i = 0; i < 3; ++i) if (p->tw[i]) (void)hipFree(p->tw[i]);
+  if (p->work_a) (void)hipFree(p->work_a);
+  if (p->work_b) (void)hipFree(p->work_b);
+  if (p->partial) (void)hipFree(p->partial);
+  if (p->counters) (void)hipFree(p->counters);
+  p->tw[0] = p->tw[1] = p->tw[2] = nullptr;
+  p->work_a = p->work_b = nullptr;
+  p->partial = nullptr;
+  p->counters = nullptr;
+}
+
+template <typename T, int MODE, int DOTS, bool GEN>
+static void launch_p5g(ConvPlanBase* p, const ConvArgs& a, hipStream_t s) {
+  const ConvGeom& g = p->g;
+  const int nblk = g.nslots * p->nwg5;
+  hipLaunchKernelGGL((k_zinv<T, MODE, DOTS, GEN>), dim3(nblk), dim3(FFT_NT), p->lds[4], s,
+                     (const C<T>*)p->work_a, (C<T>*)a.out, g, p->pz, (const C<T>*)p->tw[2], (const C<T>*)a.x,
+                     (const C<T>*)a.inv_alpha, a.mask, a.site_stride, (const C<T>*)a.b, (const C<T>*)a.d0,
+                     (const C<T>*)a.d1, p->partial, p->counters, a.ctl, a.need, p->nwg5);
+}
+
+template <typename T, int MODE, int DOTS>
+static void launch_p5(ConvPlanBase* p, const ConvArgs& a, hipStream_t s) {
+  if (p->pz.gen) launch_p5g<T, MODE, DOTS, true>(p, a, s);
+  else launch_p5g<T, MODE, DOTS, false>(p, a, s);
+}
+
+template <typename T, int MODE>
+static void launch_p5_dots(ConvPlanBase* p, const ConvArgs& a, hipStream_t s) {
+  switch (a.dots) {
+    case DOT_ALPHA: launch_p5<T, MODE, DOT_ALPHA>(p, a, s); break;
+    case DOT_ZETA: launch_p5<T, MODE, DOT_ZETA>(p, a, s); break;
+    case DOT_RESID: launch_p5<T, MODE, DOT_RESID>(p, a, s); break;
+    default: launch_p5<T, MODE, DOT_NONE>(p, a, s); break;
+  }
+}
+
+template <typename T>
+static int conv_apply_t(ConvPlanBase* p, const ConvArgs& a, hipStream_t s) {
+  const ConvGeom& g = p->g;
+  C<T>* A = (C<T>*)p->work_a;
+  C<T>* B = (C<T>*)p->work_b;
+  // P1
+  {
+    const int nblk = g.nslots * g.nx * ceil_div(g.ny, g.rows1);
+    if (p->pz.gen)
+      hipLaunchKernelGGL((k_zfwd<T, true>), dim3(nblk), dim3(FFT_NT), p->lds[0], s, (const C<T>*)a.x, A, g, p->pz,
+                         (const C<T>*)p->tw[2], a.ctl, a.need);
+    else
+      hipLaunchKernelGGL((k_zfwd<T, false>), dim3(nblk), dim3(FFT_NT), p->lds[0], s, (const C<T>*)a.x, A, g, p->pz,
+                         (const C<T>*)p->tw[2], a.ctl, a.need);
+  }
+  // P2
+  {
+    AxisArgs ax;
+    ax.n_outer = g.nslots * g.nx; ax.outer_per_slot = g.nx;
+    ax.in_rows = g.ny; ax.n_load = g.ny; ax.out_rows = g.Ly; ax.n_store = g.Ly;
+    ax.W = 3LL * g.Lz; ax.ncol = g.ncol2; ax.negate = 0;
+    const long long nblk = (long long)ax.n_outer * ceil_div(ax.W, ax.ncol);
+    if (p->py.gen)
+      hipLaunchKernelGGL((k_axis<T, false, true>), dim3((unsigned)nblk), dim3(FFT_NT), p->lds[1], s, (const C<T>*)A,
+                         B, ax, p->py, (const C<T>*)p->tw[1], a.ctl, a.need);
+    else
+      hipLaunchKernelGGL((k_axis<T, false, false>), dim3((unsigned)nblk), dim3(FFT_NT), p->lds[1], s, (const C<T>*)A,
+                         B, ax, p->py, (const C<T>*)p->tw[1], a.ctl, a.need);
+  }
+  // P3
+  {
+    const int nblk = g.nslots * g.Ly * ceil_div(g.Lz, g.wz3);
+    const T scale = T(1.0 / ((double)g.Lx * g.Ly * g.Lz));
+    if (p->px.gen)
+      hipLaunchKernelGGL((k_xconv<T, true>), dim3(nblk), dim3(FFT_NT), p->lds[2], s, B, (const C<T>*)a.table,
+                         a.table_stride, g, p->px, (const C<T>*)p->tw[0], a.ctl, a.need, scale);
+    else
+      hipLaunchKernelGGL((k_xconv<T, false>), dim3(nblk), dim3(FFT_NT), p->lds[2], s, B, (const C<T>*)a.table,
+                         a.table_stride, g, p->px, (const C<T>*)p->tw[0], a.ctl, a.need, scale);
+  }
+  // P4
+  {
+    AxisArgs ax;
+    ax.n_outer = g.nslots * g.nx; ax.outer_per_slot = g.nx;
+    ax.in_rows = g.Ly; ax.n_load = g.Ly; ax.out_rows = g.ny; ax.n_store = g.ny;
+    ax.W = 3LL * g.Lz; ax.ncol = g.ncol2; ax.negate = 0;
+    const long long nblk = (long long)ax.n_outer * ceil_div(ax.W, ax.ncol);
+    if (p->py.gen)
+      hipLaunchKernelGGL((k_axis<T, true, true>), dim3((unsigned)nblk), dim3(FFT_NT), p->lds[3], s, (const C<T>*)B,
+                         A, ax, p->py, (const C<T>*)p->tw[1], a.ctl, a.need);
+    else
+      hipLaunchKernelGGL((k_axis<T, true, false>), dim3((unsigned)nblk), dim3(FFT_NT), p->lds[3], s, (const C<T>*)B,
+                         A, ax, p->py, (const C<T>*)p->tw[1], a.ctl, a.need);
+  }
+  // P5
+  switch (a.mode) {
+    case MODE_FIELD: launch_p5_dots<T, MODE_FIELD>(p, a, s); break;
+    case MODE_PRECOND: launch_p5_dots<T, MODE_PRECOND>(p, a, s); break;
+    case MODE_INTERACT: launch_p5_dots<T, MODE_INTERACT>(p, a, s); break;
+    case MODE_RESIDUAL: launch_p5_dots<T, MODE_RESIDUAL>(p, a, s); break;
+    default: set_error("unknown convolution mode"); return PDDA_ERR_ARG;
+  }
+  PDDA_LAUNCH_CHECK();
+  return PDDA_OK;
+}
+
+int conv_apply(ConvPlanBase* p, const ConvArgs& a, hipStream_t s) {
+  if (!a.x || !a.out || !a.table) { set_error("conv_apply: null x/out/table"); return PDDA_ERR_ARG; }
+  if (a.mode != MODE_FIELD && !a.mask) { set_error("conv_apply: mask required"); return PDDA_ERR_ARG; }
+  if ((a.mode == MODE_INTERACT || a.mode == MODE_RESIDUAL) && !a.inv_alpha) {
+    set_error("conv_apply: inv_alpha required"); return PDDA_ERR_ARG;
+  }
+  if (a.mode == MODE_RESIDUAL && !a.b) { set_error("conv_apply: b required"); return PDDA_ERR_ARG; }
+  if (a.dots != DOT_NONE && (!a.ctl || !a.d0 || (a.dots == DOT_ZETA && !a.d1))) {
+    set_error("conv_apply: fused dots need ctl and operand vectors"); return PDDA_ERR_ARG;
+  }
+  return p->prec ? conv_apply_t<double>(p, a, s) : conv_apply_t<float>(p, a, s);
+}
+
+int green_build(int nx, int ny, int nz, double k, double d, void* out64, void* out32, hipStream_t s) {
+  if (nx <= 0 || ny <= 0 || nz <= 0) { set_error("green_build: sizes must be positive"); return PDDA_ERR_ARG; }
+  if (!out64) { set_error("green_build: complex128 output required"); return PDDA_ERR_ARG; }
+  FftPlan px, py, pz;
+  int rc;
+  if ((rc = make_fft_plan(2 * nx, &px))) return rc;
+  if ((rc = make_fft_plan(2 * ny, &py))) return rc;
+  if ((rc = make_fft_plan(2 * nz, &pz))) return rc;
+  const int cap = tile_cap<double>();
+  if (6 * 2 * nz > cap || 2 * nx > cap || 2 * ny > cap) {
+    set_error("green_build: FFT length too long for one LDS tile");
+    return PDDA_ERR_SIZE;
+  }
+  void *twx = nullptr, *twy = nullptr, *twz = nullptr, *G1 = nullptr, *G2 = nullptr;
+  auto cleanup = [&]() {
+    if (twx) (void)hipFree(twx);
+    if (twy) (void)hipFree(twy);
+    if (twz) (void)hipFree(twz);
+    if (G1) (void)hipFree(G1);
+    if (G2) (void)hipFree(G2);
+  };
+  if ((rc = upload_twiddles(2 * nx, 1, &twx)) || (rc = upload_twiddles(2 * ny, 1, &twy)) ||
+      (rc = upload_twiddles(2 * nz, 1, &twz))) { cleanup(); return rc; }
+  const long long W = 6LL * (nz + 1);
+  const size_t g1n = (size_t)(2 * nx) * (2 * ny) * W, g2n = (size_t)(2 * nx) * (ny + 1) * W;
+  if (hipMalloc(&G1, g1n * sizeof(double2)) != hipSuccess || hipMalloc(&G2, g2n * sizeof(double2)) != hipSuccess) {
+    cleanup();
+    set_error("green_build: out of device memory");
+    return PDDA_ERR_OOM;
+  }
+  // z pass with generation
+  const int Lz = 2 * nz;
+  int rows1 = std::max(1, std::min(2 * ny, cap / (6 * Lz)));
+  {
+    const long long want = std::max(1LL, (long long)(2 * nx) * (2 * ny) / 2048);
+    rows1 = (int)std::max(1LL, std::min((long long)rows1, want));
+  }
+  const size_t lds1 = (size_t)(Lz + Lz * 6 * rows1) * sizeof(double2);
+  const int nb1 = (2 * nx) * ceil_div(2 * ny, rows1);
+  hipLaunchKernelGGL(k_green_z, dim3(nb1), dim3(FFT_NT), lds1, s, (double2*)G1, nx, ny, nz, k, d, rows1, pz,
+                     (const double2*)twz);
+  // y pass: (2nx, 2ny, W) -> (2nx, ny+1, W)
+  {
+    AxisArgs ax;
+    ax.n_outer = 2 * nx; ax.outer_per_slot = 2 * nx;
+    ax.in_rows = 2 * ny; ax.n_load = 2 * ny; ax.out_rows = ny + 1; ax.n_store = ny + 1;
+    ax.W = W; ax.ncol = (int)std::min<long long>(W, cap / (2 * ny)); ax.negate = 0;
+    const size_t lds = (size_t)(2 * ny + 2 * ny * ax.ncol) * sizeof(double2);
+    const long long nblk = (long long)ax.n_outer * ceil_div(ax.W, ax.ncol);
+    hipLaunchKernelGGL((k_axis<double, false, true>), dim3((unsigned)nblk), dim3(FFT_NT), lds, s, (const double2*)G1,
+                       (double2*)G2, ax, py, (const double2*)twy, (const SlotCtl*)nullptr, (int)NEED_ALL);
+  }
+  // x pass: (2nx, (ny+1)W) -> (nx+1, (ny+1)W)
+  {
+    AxisArgs ax;
+    ax.n_outer = 1; ax.outer_per_slot = 1;
+    ax.in_rows = 2 * nx; ax.n_load = 2 * nx; ax.out_rows = nx + 1; ax.n_store = nx + 1;
+    ax.W = (long long)(ny + 1) * W; ax.ncol = (int)std::min<long long>(ax.W, std::min(cap / (2 * nx), 64));
+    ax.negate = 0;
+    const size_t lds = (size_t)(2 * nx + 2 * nx * ax.ncol) * sizeof(double2);
+    const long long nblk = ceil_div(ax.W, ax.ncol);
+    hipLaunchKernelGGL((k_axis<double, false, true>), dim3((unsigned)nblk), dim3(FFT_NT), lds, s, (const double2*)G2,
+                       (double2*)out64, ax, px, (const double2*)twx, (const SlotCtl*)nullptr, (int)NEED_ALL);
+  }
+  if (out32) {
+    const long long n = (long long)(nx + 1) * (ny + 1) * W;
+    hipLaunchKernelGGL(k_cast64to32, dim3(1024), dim3(256), 0, s, (const double2*)out64, (float2*)out32, n);
+  }
+  hipError_t e = hipGetLastError();
+  // the temporaries must outlive the kernels that use them
+  hipError_t e2 = hipStreamSynchronize(s);
+  cleanup();
+  if (e != hipSuccess || e2 != hipSuccess) {
+    set_error(std::string("green_build: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+    return PDDA_ERR_HIP;
+  }
+  return PDDA_OK;
+}
+
+int precond_build(int hx, int hy, int hz, double gre, double gim, const void* g64, void* out64, void* out32,
+                  hipStream_t s) {
+  if (!g64 || !out64) { set_error("precond_build: null table"); return PDDA_ERR_ARG; }
+  const long long n = (long long)(hx + 1) * (hy + 1) * (hz + 1);
+  hipLaunchKernelGGL(k_precond_invert, dim3(1024), dim3(256), 0, s, (const double2*)g64, (double2*)out64, n, gre,
+                     gim);
+  if (out32) hipLaunchKernelGGL(k_cast64to32, dim3(1024), dim3(256), 0, s, (const double2*)out64, (float2*)out32, 6 * n);
+  PDDA_LAUNCH_CHECK();
+  return PDDA_OK;
+}
+
+}  // namespace pdda

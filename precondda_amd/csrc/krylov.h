@@ -1,0 +1,91 @@
+// Host-side solver objects around the device-resident Krylov state.
+#pragma once
+
+#include <vector>
+
+#include "conv.h"
+
+namespace pdda {
+
+template <typename T> struct VecSet {
+  C<T>* b = nullptr;     // E_inc * mask
+  C<T>* r = nullptr;     // r0 (recursive residual)
+  C<T>* rhat = nullptr;  // shadow residual
+  C<T>* p = nullptr;
+  C<T>* x = nullptr;     // p_current (the preconditioned iterate y, P = M y)
+  C<T>* best = nullptr;  // best p_current so far
+  C<T>* w = nullptr;
+  C<T>* u = nullptr;
+  C<T>* z = nullptr;
+  C<T>* s0 = nullptr;    // s / s_prev ping-pong
+  C<T>* s1 = nullptr;
+  C<T>* v = nullptr;
+  C<T>* t = nullptr;
+  C<T>* y = nullptr;
+  C<T>* phat = nullptr;
+  C<T>* shat = nullptr;
+  C<T>* tmp = nullptr;
+};
+
+template <typename T> struct SolverT {
+  int nx = 0, ny = 0, nz = 0, px = 0, py = 0, pz = 0, nslots = 0, precon = 0;
+  long long n3 = 0;
+  int nwg = 0;
+  ConvPlanBase mv, pc;
+  VecSet<T> vs;
+  SlotCtl* ctl = nullptr;
+  SlotCtl* ctl_host = nullptr;
+  cd* partial = nullptr;
+  int* counters = nullptr;
+  int sidx = 0;
+  long long host_iter = 0;
+  bool lockstep = true;  // all slots started together: replacement only at iteration 1
+  // bound operands (device pointers, plan precision)
+  const C<T>* ghat = nullptr;
+  long long ghat_stride = 0;
+  const C<T>* mhat = nullptr;
+  long long mhat_stride = 0;
+  const C<T>* inv_alpha = nullptr;
+  const uint8_t* mask = nullptr;
+  long long site_stride = 0;
+
+  int init(int nx, int ny, int nz, int px, int py, int pz, int nslots, int precon);
+  void release();
+  int load(int slot, const void* E, double target, int max_iter, double ratio, int stagnation, hipStream_t s);
+  int iterate(int n_iter, hipStream_t s);
+  int status(hipStream_t s);
+  int result(void* out, hipStream_t s);
+  int conv(ConvPlanBase* plan, int mode, const C<T>* x, C<T>* out, const C<T>* table, long long tstride, int dots,
+           const C<T>* d0, const C<T>* d1, int need, hipStream_t s);
+};
+
+// complex64 GPBiCG inner solves + FP64 true-residual refinement.
+struct MixedSolver {
+  SolverT<float> inner;
+  ConvPlanBase mv64;
+  int nslots = 0;
+  long long n3 = 0;
+  int nwg = 0;
+  double2* b64 = nullptr;
+  double2* x64 = nullptr;
+  double2* r64 = nullptr;
+  float2* e32 = nullptr;
+  float2* d32 = nullptr;
+  SlotCtl* ctl64 = nullptr;
+  SlotCtl* ctl64_host = nullptr;
+  cd* partial = nullptr;
+  int* counters = nullptr;
+  const double2* ghat64 = nullptr;
+  long long ghat64_stride = 0;
+  const double2* inv_alpha64 = nullptr;
+  std::vector<int> outer_iters, inner_iters;
+  std::vector<double> final_rel;
+
+  int init(int nx, int ny, int nz, int px, int py, int pz, int nslots, int precon);
+  void release();
+  int residual64(hipStream_t s);
+  int solve(const void* E64, double tol, double inner_target, int max_outer, int max_inner, double ratio,
+            int stagnation, int check_every, hipStream_t s);
+};
+
+}  // namespace pdda

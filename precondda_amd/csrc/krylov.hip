@@ -1,0 +1,635 @@
+// Device-resident preconditioned GPBiCG (the reference's right-preconditioned
+// GPBiCGSTAB with its reset and residual-replacement rules) for a batch of
+// right-hand sides ("slots"), plus the complex64 + FP64 refinement driver.
+#include "krylov.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace pdda {
+
+constexpr int VNT = 256;  // threads per workgroup of the vector kernels
+
+// ---------------------------------------------------------------------------------
+// Reductions for vector kernels (same ticket scheme as the convolution epilogue).
+// ---------------------------------------------------------------------------------
+enum : int { FIN_STORE = 0, FIN_INIT = 1, FIN_NORM = 2 };
+
+__device__ inline void finalize_vec(SlotCtl& c, int kind, const double* re, const double* im, double p0,
+                                    double p1, int ip0, int ip1) {
+  if (kind == FIN_STORE) {
+    c.sums[0] = cd{re[0], im[0]};
+    c.sums[1] = cd{re[1], im[1]};
+  } else if (kind == FIN_INIT) {
+    // p0 = target, p1 = ratio, ip0 = max_iter, ip1 = stagnation window
+    const double n0 = sqrt(re[0]);
+    c.initial_norm = n0;
+    c.current_norm = n0;
+    c.lowest_norm = n0;
+    c.previous_lowest = n0;
+    c.first_norm = 0.0;
+    c.rho1 = cd{re[0], 0.0};
+    c.rho0 = c.rho1;
+    c.alpha = c.beta = c.zeta = c.eta = cd{0.0, 0.0};
+    c.target = p0;
+    c.ratio = p1;
+    c.max_iter = ip0;
+    c.stagnation = ip1;
+    c.it = 0;
+    c.last_change = 0;
+    c.resets = 0;
+    c.stop_it = -1;
+    c.sums[0] = cd{re[0], 0.0};
+    c.sums[1] = cd{re[0], 0.0};
+    if (ip0 <= 0) {
+      c.flags = 0;
+      c.status = ST_MAXITER;
+    } else {
+      c.flags = F_ACTIVE;
+      c.status = ST_RUNNING;
+    }
+  } else {  // FIN_NORM: refinement residual norm
+    c.sums[0] = cd{re[0], 0.0};
+    c.current_norm = sqrt(re[0]);
+  }
+}
+
+template <int ND>
+__device__ inline void vec_grid_reduce(double (&re)[ND], double (&im)[ND], cd* partial, int nwg, int wg,
+                                       int* counter, SlotCtl* ctl, int kind, double p0 = 0, double p1 = 0,
+                                       int ip0 = 0, int ip1 = 0) {
+  __shared__ double scratch[2 * ND * (VNT / 64)];
+  __shared__ int is_last;
+  block_sum<ND>(re, im, scratch);
+  if (threadIdx.x == 0) {
+    for (int d = 0; d < ND; ++d) partial[(size_t)wg * ND + d] = cd{re[d], im[d]};
+    __threadfence();
+    const int t = atomicAdd(counter, 1);
+    is_last = (t == nwg - 1);
+  }
+  __syncthreads();
+  if (!is_last) return;
+  __threadfence();
+  const volatile double* pv = reinterpret_cast<const volatile double*>(partial);
+  double r2[ND], i2[ND];
+  for (int d = 0; d < ND; ++d) { r2[d] = 0.0; i2[d] = 0.0; }
+  for (int w = threadIdx.x; w < nwg; w += blockDim.x)
+    for (int d = 0; d < ND; ++d) {
+      r2[d] += pv[2 * ((size_t)w * ND + d)];
+      i2[d] += pv[2 * ((size_t)w * ND + d) + 1];
+    }
+  block_sum<ND>(r2, i2, scratch);
+  if (threadIdx.x == 0) {
+    finalize_vec(*ctl, kind, r2, i2, p0, p1, ip0, ip1);
+    atomicExch(counter, 0);
+  }
+}
+
+template <typename T> __device__ inline C<T> cz() { return mk<T>(0, 0); }
+template <typename T> __device__ inline C<T> smul(cd s, C<T> a) {
+  return mk<T>((T)s.x * a.x - (T)s.y * a.y, (T)s.x * a.y + (T)s.y * a.x);
+}
+
+// b = E * mask; r = rhat = p = b; x = best = w = u = z = s0 = s1 = 0;
+// rho1 = <b,b>, initial norm = ||b||.
+template <typename T>
+__global__ __launch_bounds__(VNT) void k_init(const C<T>* __restrict__ E, const uint8_t* __restrict__ mask,
+                                              long long site_stride, VecSet<T> v, long long n3, int slot0,
+                                              cd* partial, int* counters, SlotCtl* ctl, double target,
+                                              double ratio, int max_iter, int stagnation, int nwg) {
+  const int slot = slot0 + blockIdx.y;
+  const long long off = (long long)slot * n3;
+  const uint8_t* m = mask + (long long)slot * site_stride;
+  const C<T>* e = E + (long long)blockIdx.y * n3;
+  double re[1] = {0.0}, im[1] = {0.0};
+  for (long long i = blockIdx.x * (long long)VNT + threadIdx.x; i < n3; i += (long long)nwg * VNT) {
+    const C<T> ev = e[i];
+    const T mv = m[i / 3] ? T(1) : T(0);
+    const C<T> b = cscale(ev, mv);
+    v.b[off + i] = b;
+    v.r[off + i] = b;
+    v.rhat[off + i] = b;
+    v.p[off + i] = b;
+    v.x[off + i] = cz<T>();
+    v.best[off + i] = cz<T>();
+    v.w[off + i] = cz<T>();
+    v.u[off + i] = cz<T>();
+    v.z[off + i] = cz<T>();
+    v.s0[off + i] = cz<T>();
+    v.s1[off + i] = cz<T>();
+    re[0] += (double)b.x * b.x + (double)b.y * b.y;
+  }
+  vec_grid_reduce<1>(re, im, partial + (size_t)slot * nwg * 6, nwg, blockIdx.x, counters + slot, ctl + slot,
+                     FIN_INIT, target, ratio, max_iter, stagnation);
+}
+
+// s = r - alpha v ; y = s_prev - s - alpha w
+template <typename T>
+__global__ __launch_bounds__(VNT) void k_upd1(const C<T>* __restrict__ r, const C<T>* __restrict__ v,
+                                              C<T>* __restrict__ s, const C<T>* __restrict__ sp,
+                                              const C<T>* __restrict__ w, C<T>* __restrict__ y, long long n3,
+                                              const SlotCtl* ctl, int nwg) {
+  const int slot = blockIdx.y;
+  if (!(ctl[slot].flags & F_ACTIVE)) return;
+  const cd al = ctl[slot].alpha;
+  const long long off = (long long)slot * n3;
+  for (long long i = blockIdx.x * (long long)VNT + threadIdx.x; i < n3; i += (long long)nwg * VNT) {
+    const long long j = off + i;
+    const C<T> sv = csub(r[j], smul<T>(al, v[j]));
+    s[j] = sv;
+    y[j] = csub(csub(sp[j], sv), smul<T>(al, w[j]));
+  }
+}
+
+// zeta-only step (first / post-reset):  u = zeta v; z = zeta s; r = s - zeta t
+// general step: u = zeta v + eta (s_prev - r + beta u); z = zeta r + eta z - alpha u;
+//               r = s - eta y - zeta t
+// x += alpha p + z ; partial sums ||r||^2, <rhat, r>
+template <typename T>
+__global__ __launch_bounds__(VNT) void k_upd2(VecSet<T> v, const C<T>* __restrict__ s, const C<T>* __restrict__ sp,
+                                              const C<T>* __restrict__ y, long long n3, cd* partial,
+                                              int* counters, SlotCtl* ctl, int nwg) {
+  const int slot = blockIdx.y;
+  const SlotCtl& c = ctl[slot];
+  if (!(c.flags & F_ACTIVE)) return;
+  const bool zonly = (c.it == 0) || (c.flags & F_WAS_RESET);
+  const cd ze = c.zeta, et = c.eta, al = c.alpha, be = c.beta;
+  const long long off = (long long)slot * n3;
+  double re[2] = {0.0, 0.0}, im[2] = {0.0, 0.0};
+  for (long long i = blockIdx.x * (long long)VNT + threadIdx.x; i < n3; i += (long long)nwg * VNT) {
+    const long long j = off + i;
+    const C<T> vv = v.v[j], sv = s[j], tv = v.t[j], rv = v.r[j];
+    C<T> un, zn, rn;
+    if (zonly) {
+      un = smul<T>(ze, vv);
+      zn = smul<T>(ze, sv);
+      rn = csub(sv, smul<T>(ze, tv));
+    } else {
+      un = cadd(smul<T>(ze, vv), smul<T>(et, cadd(csub(sp[j], rv), smul<T>(be, v.u[j]))));
+      zn = csub(cadd(smul<T>(ze, rv), smul<T>(et, v.z[j])), smul<T>(al, un));
+      rn = csub(csub(sv, smul<T>(et, y[j])), smul<T>(ze, tv));
+    }
+    v.u[j] = un;
+    v.z[j] = zn;
+    v.r[j] = rn;
+    v.x[j] = cadd(cadd(v.x[j], smul<T>(al, v.p[j])), zn);
+    const C<T> h = v.rhat[j];
+    re[0] += (double)rn.x * rn.x + (double)rn.y * rn.y;
+    re[1] += (double)h.x * rn.x + (double)h.y * rn.y;
+    im[1] += (double)h.x * rn.y - (double)h.y * rn.x;
+  }
+  vec_grid_reduce<2>(re, im, partial + (size_t)slot * nwg * 6, nwg, blockIdx.x, counters + slot, ctl + slot,
+                     FIN_STORE);
+}
+
+// current_norm = sqrt(<r,r>) for the slots whose residual was just recomputed.
+__global__ void k_norm_from_sums(SlotCtl* ctl, int nslots) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < nslots && (ctl[s].flags & F_ACTIVE)) ctl[s].current_norm = sqrt(ctl[s].sums[0].x);
+}
+
+// Per-slot control step: norm checks, best-iterate flag, stop, rho update, reset decision.
+__global__ void k_control(SlotCtl* ctl, int nslots) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nslots) return;
+  SlotCtl& c = ctl[s];
+  c.flags &= ~(F_RESET | F_RECOMPUTE | F_IMPROVED);
+  if (!(c.flags & F_ACTIVE)) return;
+  const int i = c.it;
+  const double cur = sqrt(c.sums[0].x);
+  const cd rdot = c.sums[1];
+  c.current_norm = cur;
+  if (i == 0) c.first_norm = cur;
+  c.last_change += 1;
+  if (!isnan(cur) && (isnan(c.lowest_norm) || cur < c.lowest_norm)) {
+    c.last_change = 0;
+    c.lowest_norm = cur;
+    c.flags |= F_IMPROVED;
+  }
+  if (isnan(cur) || c.last_change >= c.stagnation) {
+    c.flags &= ~F_ACTIVE;
+    c.status = ST_FAILED;
+    c.stop_it = i;
+    return;
+  }
+  if (cur < c.target * c.initial_norm) {
+    c.flags &= ~F_ACTIVE;
+    c.status = ST_CONVERGED;
+    c.stop_it = i;
+    return;
+  }
+  c.rho0 = c.rho1;
+  c.rho1 = rdot;
+  bool reset = false;
+  if (c.ratio != 0.0) {
+    if (i == 2 || (i > 2 && cur < c.previous_lowest / c.ratio)) {
+      reset = true;
+      c.previous_lowest = cur;
+      c.last_change = 0;
+    }
+  }
+  if (reset) {
+    c.flags |= F_RESET | F_WAS_RESET;
+    if (i % 100 != 0) c.flags |= F_RECOMPUTE;
+    c.beta = cd{0.0, 0.0};
+    c.resets += 1;
+  } else {
+    c.flags &= ~F_WAS_RESET;
+    c.beta = cd_div(cd_mul(cd_div(c.alpha, c.zeta), c.rho1), c.rho0);
+  }
+  c.it = i + 1;
+  if (c.it >= c.max_iter) {
+    c.flags &= ~(F_ACTIVE | F_RESET | F_RECOMPUTE);
+    c.status = ST_MAXITER;
+    c.stop_it = i;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(VNT) void k_copy_best(const C<T>* __restrict__ x, C<T>* __restrict__ best, long long n3,
+                                                   const SlotCtl* ctl, int nwg) {
+  const int slot = blockIdx.y;
+  if (!(ctl[slot].flags & F_IMPROVED)) return;
+  const long long off = (long long)slot * n3;
+  for (long long i = blockIdx.x * (long long)VNT + threadIdx.x; i < n3; i += (long long)nwg * VNT)
+    best[off + i] = x[off + i];
+}
+
+// reset: rhat = r; w = 0; p = r        else: w = t + beta v; p = r + beta (p - u)
+template <typename T>
+__global__ __launch_bounds__(VNT) void k_upd3(VecSet<T> v, long long n3, const SlotCtl* ctl, int nwg) {
+  const int slot = blockIdx.y;
+  const SlotCtl& c = ctl[slot];
+  if (!(c.flags & F_ACTIVE)) return;
+  const bool reset = (c.flags & F_RESET) != 0;
+  const cd be = c.beta;
+  const long long off = (long long)slot * n3;
+  for (long long i = blockIdx.x * (long long)VNT + threadIdx.x; i < n3; i += (long long)nwg * VNT) {
+    const long long j = off + i;
+    const C<T> rv = v.r[j];
+    if (reset) {
+      v.rhat[j] = rv;
+      v.w[j] = cz<T>();
+      v.p[j] = rv;
+    } else {
+      v.w[j] = cadd(v.t[j], smul<T>(be, v.v[j]));
+      v.p[j] = cadd(rv, smul<T>(be, csub(v.p[j], v.u[j])));
+    }
+  }
+}
+
+// out = (status == converged) ? x : best
+template <typename T>
+__global__ __launch_bounds__(VNT) void k_select(const C<T>* __restrict__ x, const C<T>* __restrict__ best,
+                                                C<T>* __restrict__ out, long long n3, const SlotCtl* ctl, int nwg) {
+  const int slot = blockIdx.y;
+  const bool conv = ctl[slot].status == ST_CONVERGED;
+  const C<T>* src = conv ? x : best;
+  const long long off = (long long)slot * n3;
+  for (long long i = blockIdx.x * (long long)VNT + threadIdx.x; i < n3; i += (long long)nwg * VNT)
+    out[off + i] = src[off + i];
+}
+
+// ---------------------------------------------------------------------------------
+// Refinement kernels (complex64 inner solve, FP64 outer residual)
+// ---------------------------------------------------------------------------------
+// E32 = r64 / ||r64|| for slots still refining; stores the scale in ctl.rhs_scale.
+__global__ __launch_bounds__(VNT) void k_refine_rhs(const double2* __restrict__ r, float2* __restrict__ e,
+                                                    long long n3, SlotCtl* ctl64, int nwg) {
+  const int slot = blockIdx.y;
+  const double nr = ctl64[slot].current_norm;
+  const double sc = nr > 0.0 ? 1.0 / nr : 0.0;
+  const long long off = (long long)slot * n3;
+  for (long long i = blockIdx.x * (long long)VNT + threadIdx.x; i < n3; i += (long long)nwg * VNT) {
+    const double2 v = r[off + i];
+    e[off + i] = make_float2((float)(v.x * sc), (float)(v.y * sc));
+  }
+}
+
+// x64 += ||r|| * d32 for slots that were refined this round (flags in ctl64.flags).
+__global__ __launch_bounds__(VNT) void k_refine_acc(double2* __restrict__ x, const float2* __restrict__ d,
+                                                    long long n3, const SlotCtl* ctl64, int nwg) {
+  const int slot = blockIdx.y;
+  if (!(ctl64[slot].flags & F_ACTIVE)) return;
+  const double sc = ctl64[slot].current_norm;
+  const long long off = (long long)slot * n3;
+  for (long long i = blockIdx.x * (long long)VNT + threadIdx.x; i < n3; i += (long long)nwg * VNT) {
+    const float2 dv = d[off + i];
+    double2 xv = x[off + i];
+    xv.x += sc * (double)dv.x;
+    xv.y += sc * (double)dv.y;
+    x[off + i] = xv;
+  }
+}
+
+// b64 = E * mask, x64 = 0; norm of b into ctl64 (initial_norm, current_norm).
+__global__ __launch_bounds__(VNT) void k_refine_init(const double2* __restrict__ E, const uint8_t* __restrict__ mask,
+                                                     long long site_stride, double2* __restrict__ b,
+                                                     double2* __restrict__ x, double2* __restrict__ r, long long n3,
+                                                     cd* partial, int* counters, SlotCtl* ctl64, int nwg) {
+  const int slot = blockIdx.y;
+  const long long off = (long long)slot * n3;
+  const uint8_t* m = mask + (long long)slot * site_stride;
+  double re[1] = {0.0}, im[1] = {0.0};
+  for (long long i = blockIdx.x * (long long)VNT + threadIdx.x; i < n3; i += (long long)nwg * VNT) {
+    const double2 ev = E[off + i];
+    const double mv = m[i / 3] ? 1.0 : 0.0;
+    const double2 bv = make_double2(ev.x * mv, ev.y * mv);
+    b[off + i] = bv;
+    r[off + i] = bv;
+    x[off + i] = make_double2(0.0, 0.0);
+    re[0] += bv.x * bv.x + bv.y * bv.y;
+  }
+  vec_grid_reduce<1>(re, im, partial + (size_t)slot * nwg * 6, nwg, blockIdx.x, counters + slot, ctl64 + slot,
+                     FIN_NORM);
+}
+
+// ---------------------------------------------------------------------------------
+// Host: Solver<T>
+// ---------------------------------------------------------------------------------
+static int vec_nwg(long long n3) {
+  long long w = (n3 + (long long)VNT * 4 - 1) / ((long long)VNT * 4);
+  return (int)std::max(1LL, std::min(w, 1024LL));
+}
+
+template <typename T>
+int SolverT<T>::init(int nx_, int ny_, int nz_, int px_, int py_, int pz_, int nslots_, int precon_) {
+  nx = nx_; ny = ny_; nz = nz_; nslots = nslots_; precon = precon_;
+  px = px_ > 0 ? px_ : nx; py = py_ > 0 ? py_ : ny; pz = pz_ > 0 ? pz_ : nz;
+  n3 = 3LL * nx * ny * nz;
+  const int prec = sizeof(T) == 8 ? 1 : 0;
+  int rc;
+  if ((rc = conv_plan_init(&mv, prec, nx, ny, nz, nx, ny, nz, nslots))) return rc;
+  if (precon && (rc = conv_plan_init(&pc, prec, nx, ny, nz, px, py, pz, nslots))) return rc;
+  nwg = vec_nwg(n3);
+  const size_t vb = (size_t)n3 * nslots * sizeof(C<T>);
+  C<T>** bufs[] = {&vs.b, &vs.r, &vs.rhat, &vs.p, &vs.x, &vs.best, &vs.w, &vs.u, &vs.z, &vs.s0, &vs.s1,
+                   &vs.v, &vs.t, &vs.y, &vs.phat, &vs.shat, &vs.tmp};
+  for (C<T>** b : bufs) PDDA_HIP_CHECK(hipMalloc((void**)b, vb));
+  PDDA_HIP_CHECK(hipMalloc(&ctl, nslots * sizeof(SlotCtl)));
+  PDDA_HIP_CHECK(hipMemset(ctl, 0, nslots * sizeof(SlotCtl)));
+  PDDA_HIP_CHECK(hipHostMalloc((void**)&ctl_host, nslots * sizeof(SlotCtl), hipHostMallocDefault));
+  std::memset(ctl_host, 0, nslots * sizeof(SlotCtl));
+  PDDA_HIP_CHECK(hipMalloc(&partial, (size_t)nslots * nwg * 6 * sizeof(cd)));
+  PDDA_HIP_CHECK(hipMalloc(&counters, nslots * sizeof(int)));
+  PDDA_HIP_CHECK(hipMemset(counters, 0, nslots * sizeof(int)));
+  return PDDA_OK;
+}
+
+template <typename T>
+void SolverT<T>::release() {
+  conv_plan_free(&mv);
+  if (precon) conv_plan_free(&pc);
+  C<T>* bufs[] = {vs.b, vs.r, vs.rhat, vs.p, vs.x, vs.best, vs.w, vs.u, vs.z, vs.s0, vs.s1,
+                  vs.v, vs.t, vs.y, vs.phat, vs.shat, vs.tmp};
+  for (C<T>* b : bufs) if (b) (void)hipFree(b);
+  if (ctl) (void)hipFree(ctl);
+  if (ctl_host) (void)hipHostFree(ctl_host);
+  if (partial) (void)hipFree(partial);
+  if (counters) (void)hipFree(counters);
+}
+
+template <typename T>
+int SolverT<T>::load(int slot, const void* E, double target, int max_iter, double ratio, int stagnation,
+                     hipStream_t s) {
+  if (!mask) { set_error("solver: operands not bound"); return PDDA_ERR_STATE; }
+  const int s0 = slot < 0 ? 0 : slot;
+  const int ns = slot < 0 ? nslots : 1;
+  if (s0 + ns > nslots) { set_error("solver: slot out of range"); return PDDA_ERR_ARG; }
+  hipLaunchKernelGGL(k_init<T>, dim3(nwg, ns), dim3(VNT), 0, s, (const C<T>*)E, mask, site_stride, vs, n3, s0,
+                     partial, counters, ctl, target, ratio, max_iter, stagnation, nwg);
+  sidx = 0;
+  host_iter = 0;
+  PDDA_LAUNCH_CHECK();
+  return PDDA_OK;
+}
+
+template <typename T>
+int SolverT<T>::conv(ConvPlanBase* plan, int mode, const C<T>* x, C<T>* out, const C<T>* table, long long tstride,
+                     int dots, const C<T>* d0, const C<T>* d1, int need, hipStream_t s) {
+  ConvArgs a;
+  a.mode = mode;
+  a.x = x;
+  a.out = out;
+  a.table = table;
+  a.table_stride = tstride;
+  a.inv_alpha = inv_alpha;
+  a.mask = mask;
+  a.site_stride = site_stride;
+  a.b = vs.b;
+  a.dots = dots;
+  a.d0 = d0;
+  a.d1 = d1;
+  a.ctl = ctl;
+  a.need = need;
+  return conv_apply(plan, a, s);
+}
+
+template <typename T>
+int SolverT<T>::iterate(int n_iter, hipStream_t s) {
+  int rc;
+  const dim3 vg(nwg, nslots);
+  for (int k = 0; k < n_iter; ++k) {
+    C<T>* s_new = sidx ? vs.s0 : vs.s1;
+    C<T>* s_prev = sidx ? vs.s1 : vs.s0;
+    const C<T>* ph = vs.p;
+    if (precon) {
+      if ((rc = conv(&pc, MODE_PRECOND, vs.p, vs.phat, mhat, mhat_stride, DOT_NONE, nullptr, nullptr, NEED_ACTIVE, s)))
+        return rc;
+      ph = vs.phat;
+    }
+    if ((rc = conv(&mv, MODE_INTERACT, ph, vs.v, ghat, ghat_stride, DOT_ALPHA, vs.rhat, nullptr, NEED_ACTIVE, s)))
+      return rc;
+    hipLaunchKernelGGL(k_upd1<T>, vg, dim3(VNT), 0, s, vs.r, vs.v, s_new, s_prev, vs.w, vs.y, n3, ctl, nwg);
+    const C<T>* sh = s_new;
+    if (precon) {
+      if ((rc = conv(&pc, MODE_PRECOND, s_new, vs.shat, mhat, mhat_stride, DOT_NONE, nullptr, nullptr, NEED_ACTIVE, s)))
+        return rc;
+      sh = vs.shat;
+    }
+    if ((rc = conv(&mv, MODE_INTERACT, sh, vs.t, ghat, ghat_stride, DOT_ZETA, vs.y, s_new, NEED_ACTIVE, s)))
+      return rc;
+    hipLaunchKernelGGL(k_upd2<T>, vg, dim3(VNT), 0, s, vs, s_new, s_prev, vs.y, n3, partial, counters, ctl, nwg);
+    // true-residual replacement at the slot's iteration 1
+    if (!lockstep || host_iter == 1) {
+      const C<T>* xs = vs.x;
+      if (precon) {
+        if ((rc = conv(&pc, MODE_PRECOND, vs.x, vs.phat, mhat, mhat_stride, DOT_NONE, nullptr, nullptr, NEED_REPLACE, s)))
+          return rc;
+        xs = vs.phat;
+      }
+      if ((rc = conv(&mv, MODE_RESIDUAL, xs, vs.r, ghat, ghat_stride, DOT_RESID, vs.rhat, nullptr, NEED_REPLACE, s)))
+        return rc;
+    }
+    hipLaunchKernelGGL(k_control, dim3((nslots + 63) / 64), dim3(64), 0, s, ctl, nslots);
+    hipLaunchKernelGGL(k_copy_best<T>, vg, dim3(VNT), 0, s, vs.x, vs.best, n3, ctl, nwg);
+    // reset with residual recompute
+    {
+      const C<T>* xs = vs.x;
+      if (precon) {
+        if ((rc = conv(&pc, MODE_PRECOND, vs.x, vs.phat, mhat, mhat_stride, DOT_NONE, nullptr, nullptr, NEED_RECOMPUTE, s)))
+          return rc;
+        xs = vs.phat;
+      }
+      if ((rc = conv(&mv, MODE_RESIDUAL, xs, vs.r, ghat, ghat_stride, DOT_NONE, nullptr, nullptr, NEED_RECOMPUTE, s)))
+        return rc;
+    }
+    hipLaunchKernelGGL(k_upd3<T>, vg, dim3(VNT), 0, s, vs, n3, ctl, nwg);
+    sidx ^= 1;
+    ++host_iter;
+  }
+  PDDA_LAUNCH_CHECK();
+  return PDDA_OK;
+}
+
+template <typename T>
+int SolverT<T>::status(hipStream_t s) {
+  PDDA_HIP_CHECK(hipMemcpyAsync(ctl_host, ctl, nslots * sizeof(SlotCtl), hipMemcpyDeviceToHost, s));
+  PDDA_HIP_CHECK(hipStreamSynchronize(s));
+  return PDDA_OK;
+}
+
+template <typename T>
+int SolverT<T>::result(void* out, hipStream_t s) {
+  const dim3 vg(nwg, nslots);
+  hipLaunchKernelGGL(k_select<T>, vg, dim3(VNT), 0, s, vs.x, vs.best, vs.tmp, n3, ctl, nwg);
+  if (precon) {
+    int rc = conv(&pc, MODE_PRECOND, vs.tmp, (C<T>*)out, mhat, mhat_stride, DOT_NONE, nullptr, nullptr, NEED_ALL, s);
+    if (rc) return rc;
+  } else {
+    PDDA_HIP_CHECK(hipMemcpyAsync(out, vs.tmp, (size_t)n3 * nslots * sizeof(C<T>), hipMemcpyDeviceToDevice, s));
+  }
+  PDDA_LAUNCH_CHECK();
+  return PDDA_OK;
+}
+
+template struct SolverT<float>;
+template struct SolverT<double>;
+
+// ---------------------------------------------------------------------------------
+// Host: mixed-precision refinement
+// ---------------------------------------------------------------------------------
+int MixedSolver::init(int nx, int ny, int nz, int px, int py, int pz, int nslots_, int precon) {
+  nslots = nslots_;
+  int rc;
+  if ((rc = inner.init(nx, ny, nz, px, py, pz, nslots, precon))) return rc;
+  if ((rc = conv_plan_init(&mv64, 1, nx, ny, nz, nx, ny, nz, nslots))) return rc;
+  n3 = inner.n3;
+  nwg = vec_nwg(n3);
+  const size_t vb = (size_t)n3 * nslots * sizeof(double2);
+  PDDA_HIP_CHECK(hipMalloc(&b64, vb));
+  PDDA_HIP_CHECK(hipMalloc(&x64, vb));
+  PDDA_HIP_CHECK(hipMalloc(&r64, vb));
+  PDDA_HIP_CHECK(hipMalloc(&e32, (size_t)n3 * nslots * sizeof(float2)));
+  PDDA_HIP_CHECK(hipMalloc(&d32, (size_t)n3 * nslots * sizeof(float2)));
+  PDDA_HIP_CHECK(hipMalloc(&ctl64, nslots * sizeof(SlotCtl)));
+  PDDA_HIP_CHECK(hipMemset(ctl64, 0, nslots * sizeof(SlotCtl)));
+  PDDA_HIP_CHECK(hipHostMalloc((void**)&ctl64_host, nslots * sizeof(SlotCtl), hipHostMallocDefault));
+  PDDA_HIP_CHECK(hipMalloc(&partial, (size_t)nslots * nwg * 6 * sizeof(cd)));
+  PDDA_HIP_CHECK(hipMalloc(&counters, nslots * sizeof(int)));
+  PDDA_HIP_CHECK(hipMemset(counters, 0, nslots * sizeof(int)));
+  return PDDA_OK;
+}
+
+void MixedSolver::release() {
+  inner.release();
+  conv_plan_free(&mv64);
+  void* bufs[] = {b64, x64, r64, e32, d32, ctl64, partial, counters};
+  for (void* b : bufs) if (b) (void)hipFree(b);
+  if (ctl64_host) (void)hipHostFree(ctl64_host);
+}
+
+int MixedSolver::residual64(hipStream_t s) {
+  ConvArgs a;
+  a.mode = MODE_RESIDUAL;
+  a.x = x64;
+  a.out = r64;
+  a.table = ghat64;
+  a.table_stride = ghat64_stride;
+  a.inv_alpha = inv_alpha64;
+  a.mask = inner.mask;
+  a.site_stride = inner.site_stride;
+  a.b = b64;
+  a.dots = DOT_RESID;
+  a.d0 = r64;
+  a.ctl = ctl64;
+  a.need = NEED_ACTIVE;
+  int rc = conv_apply(&mv64, a, s);
+  if (rc) return rc;
+  // current_norm = sqrt(sums[0]) for the refined slots
+  hipLaunchKernelGGL(k_norm_from_sums, dim3((nslots + 63) / 64), dim3(64), 0, s, ctl64, nslots);
+  PDDA_LAUNCH_CHECK();
+  return PDDA_OK;
+}
+
+int MixedSolver::solve(const void* E64, double tol, double inner_target, int max_outer, int max_inner,
+                       double ratio, int stagnation, int check_every, hipStream_t s) {
+  int rc;
+  const dim3 vg(nwg, nslots);
+  hipLaunchKernelGGL(k_refine_init, vg, dim3(VNT), 0, s, (const double2*)E64, inner.mask, inner.site_stride, b64,
+                     x64, r64, n3, partial, counters, ctl64, nwg);
+  PDDA_LAUNCH_CHECK();
+  PDDA_HIP_CHECK(hipMemcpyAsync(ctl64_host, ctl64, nslots * sizeof(SlotCtl), hipMemcpyDeviceToHost, s));
+  PDDA_HIP_CHECK(hipStreamSynchronize(s));
+  std::vector<double> bnorm(nslots);
+  std::vector<int> done(nslots, 0);
+  outer_iters.assign(nslots, 0);
+  inner_iters.assign(nslots, 0);
+  final_rel.assign(nslots, 0.0);
+  for (int q = 0; q < nslots; ++q) {
+    bnorm[q] = ctl64_host[q].current_norm;
+    ctl64_host[q].initial_norm = bnorm[q];
+    done[q] = !(bnorm[q] > 0.0);
+    ctl64_host[q].flags = done[q] ? 0 : F_ACTIVE;
+  }
+  PDDA_HIP_CHECK(hipMemcpyAsync(ctl64, ctl64_host, nslots * sizeof(SlotCtl), hipMemcpyHostToDevice, s));
+  for (int outer = 0; outer < max_outer; ++outer) {
+    bool any = false;
+    for (int q = 0; q < nslots; ++q) any |= !done[q];
+    if (!any) break;
+    // inner rhs = r / ||r|| for refining slots; other slots get max_iter 0 (idle)
+    hipLaunchKernelGGL(k_refine_rhs, vg, dim3(VNT), 0, s, r64, e32, n3, ctl64, nwg);
+    PDDA_LAUNCH_CHECK();
+    for (int q = 0; q < nslots; ++q) {
+      if ((rc = inner.load(q, e32 + (size_t)q * n3, inner_target, done[q] ? 0 : max_inner, ratio, stagnation, s)))
+        return rc;
+    }
+    int it_done = 0;
+    while (true) {
+      if ((rc = inner.iterate(check_every, s))) return rc;
+      it_done += check_every;
+      if ((rc = inner.status(s))) return rc;
+      bool running = false;
+      for (int q = 0; q < nslots; ++q) running |= (inner.ctl_host[q].flags & F_ACTIVE) != 0;
+      if (!running) break;
+    }
+    for (int q = 0; q < nslots; ++q)
+      if (!done[q]) {
+        const SlotCtl& c = inner.ctl_host[q];
+        inner_iters[q] += c.stop_it + 1;
+      }
+    if ((rc = inner.result(d32, s))) return rc;
+    hipLaunchKernelGGL(k_refine_acc, vg, dim3(VNT), 0, s, x64, d32, n3, ctl64, nwg);
+    PDDA_LAUNCH_CHECK();
+    if ((rc = residual64(s))) return rc;
+    PDDA_HIP_CHECK(hipMemcpyAsync(ctl64_host, ctl64, nslots * sizeof(SlotCtl), hipMemcpyDeviceToHost, s));
+    PDDA_HIP_CHECK(hipStreamSynchronize(s));
+    for (int q = 0; q < nslots; ++q) {
+      if (done[q]) continue;
+      outer_iters[q] = outer + 1;
+      const double rel = ctl64_host[q].current_norm / bnorm[q];
+      final_rel[q] = rel;
+      if (rel < tol || std::isnan(rel)) {
+        done[q] = 1;
+        ctl64_host[q].flags = 0;
+      }
+    }
+    PDDA_HIP_CHECK(hipMemcpyAsync(ctl64, ctl64_host, nslots * sizeof(SlotCtl), hipMemcpyHostToDevice, s));
+  }
+  PDDA_HIP_CHECK(hipStreamSynchronize(s));
+  return PDDA_OK;
+}
+
+}  // namespace pdda

@@ -1,0 +1,99 @@
+"""HIP operators against the CPU oracle: Green's table, FFT matvec, preconditioner,
+excited field, Krylov primitives, cross sections."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import dda_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _sphere(R, m=1.5 + 0.1j, x=1.0):
+    pb, _ = O.sphere_problem(R, m, x)
+    return pb
+
+
+@pytest.mark.parametrize("n", [(4, 4, 4), (6, 5, 4), (8, 8, 16), (7, 6, 5), (16, 16, 16), (12, 24, 48)])
+def test_green_table_matches_oracle(n):
+    import precondda_amd as P
+    k, d = 0.31, 1.0
+    g64 = P.generate_interaction_row(*n, k, d, double_precision=True).cpu().numpy()
+    ref = O.generate_interaction_row(*n, k, d)
+    err = np.abs(g64 - ref).max() / np.abs(ref).max()
+    print(n, "green rel err", err)
+    assert err < 1e-13
+    g32 = P.generate_interaction_row(*n, k, d, double_precision=False).cpu().numpy()
+    assert np.abs(g32 - ref).max() / np.abs(ref).max() < 1e-6
+
+
+@pytest.mark.parametrize("R", [3, 4, 8])
+def test_interaction_mult_matches_dense(R):
+    import precondda_amd as P
+    pb = _sphere(R)
+    grid, k = pb["grid_size"], pb["k"]
+    ia, mask, G, M = O.prepare_interaction_matrices(grid, k, 1.0, pb["alpha_array"])
+    rng = np.random.default_rng(R)
+    n3 = int(np.prod(grid)) * 3
+    x = rng.standard_normal(n3) + 1j * rng.standard_normal(n3)
+    ref = O.interaction_mult(x, G, np.repeat(ia.ravel(), 3), mask, grid)
+    if R <= 4:
+        D, _, _ = O.dense_matrix(grid, k, 1.0, ia)
+        assert np.linalg.norm(D @ x - ref) / np.linalg.norm(ref) < 1e-13
+    Gd = torch.as_tensor(G).cuda()
+    out = P.interaction_mult(x, None, Gd, ia, mask, None, *grid).cpu().numpy()
+    err = np.linalg.norm(out - ref) / np.linalg.norm(ref)
+    print(R, "fp64 matvec err", err)
+    assert err < 1e-13
+    out32 = P.interaction_mult(x, None, Gd.to(torch.complex64), ia, mask, None, *grid).cpu().numpy()
+    err32 = np.linalg.norm(out32 - ref) / np.linalg.norm(ref)
+    print(R, "fp32 matvec err", err32)
+    assert err32 < 2e-6
+
+
+@pytest.mark.parametrize("shape", [(6, 6, 6), (10, 12, 14), (16, 16, 32)])
+def test_preconditioner_and_field_match_oracle(shape):
+    import precondda_amd as P
+    rng = np.random.default_rng(7)
+    k = 0.27
+    nx, ny, nz = shape
+    n3 = nx * ny * nz * 3
+    alpha = (0.2 + 0.01j) * (rng.random(shape) < 0.6)
+    ia, mask, G, _ = O.prepare_interaction_matrices(shape, k, 1.0, alpha, expansion=shape)
+    Mref, gam = O.build_preconditioner(shape, k, 1.0, ia, 1.0, expansion=(nx + 2, ny + 4, nz))
+    Mdev = P.build_preconditioner(nx, ny, nz, k, 1.0, ia, 1.0, expansion=(nx + 2, ny + 4, nz), double_precision=True)
+    assert np.abs(Mdev.cpu().numpy() - Mref).max() / np.abs(Mref).max() < 1e-12
+    x = rng.standard_normal(n3) + 1j * rng.standard_normal(n3)
+    ref = O.fft_matvec3(x, Mref, mask, shape)
+    out = P.fft_matvec3(x, None, Mdev, mask, None, nx, ny, nz).cpu().numpy()
+    assert np.linalg.norm(out - ref) / np.linalg.norm(ref) < 1e-13
+    fref = O.E_field_solver(x, G, shape)
+    fout = P.E_field_solver(torch.as_tensor(x).cuda(), torch.as_tensor(G).cuda(), shape).cpu().numpy()
+    assert np.linalg.norm(fout - fref) / np.linalg.norm(fref) < 1e-13
+    mref = O.fft_matvec(x, G, mask, shape)
+    mout = P.fft_matvec(x, None, torch.as_tensor(G).cuda(), mask, None, nx, ny, nz).cpu().numpy()
+    assert np.linalg.norm(mout - mref) / np.linalg.norm(mref) < 1e-13
+
+
+def test_krylov_primitives():
+    import precondda_amd as P
+    rng = np.random.default_rng(3)
+    n = 30000
+    y, t, s = (rng.standard_normal(n) + 1j * rng.standard_normal(n) for _ in range(3))
+    d = P.inner_product(torch.as_tensor(y).cuda(), torch.as_tensor(t).cuda())
+    assert abs(d - np.vdot(y, t)) / abs(np.vdot(y, t)) < 1e-13
+    ze, et = P.calculate_zeta_eta_twostep(*(torch.as_tensor(v).cuda() for v in (y, t, s)))
+    zr, er = O.calculate_zeta_eta(y, t, s)
+    assert abs(ze - zr) < 1e-12 * abs(zr) and abs(et - er) < 1e-12 * max(abs(er), 1e-300)
+
+
+def test_cross_sections_match_oracle():
+    import precondda_amd as P
+    pb = _sphere(4, 1.5 + 0.05j, 1.5)
+    grid, k = pb["grid_size"], pb["k"]
+    ia, mask, G, M = O.prepare_interaction_matrices(grid, k, 1.0, pb["alpha_array"])
+    it, Pol, _, _ = O.solve_gpbicgstab(grid, ia, G, M, pb["E_inc"], mask, 100, 500, 1e-12)
+    ce, ca = O.cross_sections(Pol, pb["E_inc"], ia, k)
+    de, da = P.cross_sections(torch.as_tensor(Pol).cuda(), pb["E_inc"], ia, k)
+    assert abs(de - ce) < 1e-12 * abs(ce)
+    assert abs(da - ca) < 1e-12 * abs(ca)

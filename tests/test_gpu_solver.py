@@ -1,0 +1,66 @@
+"""Device GPBiCGSTAB against the step-by-step complex128 oracle."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import dda_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(R, m, x, precon=True):
+    pb, _ = O.sphere_problem(R, m, x)
+    grid, k = pb["grid_size"], pb["k"]
+    ia, mask, G, M = O.prepare_interaction_matrices(grid, k, 1.0, pb["alpha_array"])
+    return pb, grid, k, ia, mask, G, M
+
+
+@pytest.mark.parametrize("precon", [True, False])
+def test_fp64_solve_matches_oracle(precon):
+    import precondda_amd as P
+    pb, grid, k, ia, mask, G, M = _setup(4, 1.5 + 0.01j, 1.0)
+    ref = O.solve_gpbicgstab(grid, ia, G, M, pb["E_inc"], mask, 100, 300, 1e-10, precon=precon)
+    out = P.solve_gpbicgstab(grid, ia, torch.as_tensor(G).cuda(), torch.as_tensor(M).cuda(), pb["E_inc"], mask,
+                             100, 300, 1e-10, precon=precon)
+    print("iters", ref[0], out[0])
+    assert out[0] == ref[0]
+    Pd = out[1].cpu().numpy()
+    err = np.linalg.norm(Pd - ref[1]) / np.linalg.norm(ref[1])
+    print("P rel err", err)
+    assert err < 1e-10
+    assert abs(out[2] - ref[2]) <= 1e-6 * abs(ref[2]) + 1e-14
+    ce, ca = O.cross_sections(ref[1], pb["E_inc"], ia, k)
+    de, da = P.cross_sections(out[1], pb["E_inc"], ia, k)
+    assert abs(de - ce) < 1e-10 * abs(ce) and abs(da - ca) < 1e-10 * abs(ca)
+
+
+def test_fp64_solve_cfg1_sphere():
+    """16^3 sphere, 2176 dipoles, m = 1.5+0.01i, x = 2, target 1e-8."""
+    import precondda_amd as P
+    pb, grid, k, ia, mask, G, M = _setup(8, 1.5 + 0.01j, 2.0)
+    ref = O.solve_gpbicgstab(grid, ia, G, M, pb["E_inc"], mask, 100, 1000, 1e-8)
+    out = P.solve_gpbicgstab(grid, ia, torch.as_tensor(G).cuda(), torch.as_tensor(M).cuda(), pb["E_inc"], mask,
+                             100, 1000, 1e-8)
+    print("iters", ref[0], out[0])
+    assert out[0] == ref[0]
+    err = np.linalg.norm(out[1].cpu().numpy() - ref[1]) / np.linalg.norm(ref[1])
+    print("P rel err", err)
+    assert err < 1e-10
+
+
+def test_mixed_precision_small():
+    import precondda_amd as P
+    pb, grid, k, ia, mask, G, M = _setup(6, 1.33 + 0.001j, 2.0)
+    ref = O.solve_gpbicgstab(grid, ia, G, M, pb["E_inc"], mask, 100, 1000, 1e-12)
+    Gd = torch.as_tensor(G).cuda()
+    Md = torch.as_tensor(M).cuda()
+    Pm, info = P.solve_mixed(grid, ia, Gd, Gd.to(torch.complex64), Md.to(torch.complex64), pb["E_inc"], mask,
+                             tol=1e-6, inner_target=1e-3)
+    print(info)
+    Pm = Pm.cpu().numpy()
+    rel = O.true_residual(Pm, pb["E_inc"], ia, mask, G, grid)
+    print("true residual", rel)
+    assert rel < 1e-6
+    ce, _ = O.cross_sections(ref[1], pb["E_inc"], ia, k)
+    me, _ = O.cross_sections(Pm, pb["E_inc"], ia, k)
+    assert abs(me - ce) < 1e-5 * abs(ce)
