@@ -94,6 +94,12 @@ long long pdda_conv_workspace_bytes(const pdda_conv* c);
 int pdda_conv_apply(pdda_conv* c, int mode, const void* x, void* out, const void* table, long long table_stride,
                     const void* inv_alpha, const void* mask, long long site_stride, const void* b, void* stream);
 
+/* Runs `reps` convolutions (shared table and site arrays) and writes the mean
+ * duration in milliseconds of each of the five passes to host ms5[5], timed with HIP
+ * events on `stream` (measurement support; no counterpart in the reference). */
+int pdda_conv_time_passes(pdda_conv* c, int mode, const void* x, void* out, const void* table,
+                          const void* inv_alpha, const void* mask, int reps, double* ms5, void* stream);
+
 /* sum conj(a) b over n complex elements; result (re, im) to host out2.
  * Replaces inner_product (Gpbicgstab.py:449-464). */
 int pdda_dot(int prec, const void* a, const void* b, long long n, double* out2, void* stream);
@@ -105,6 +111,25 @@ int pdda_zeta_eta(int prec, const void* y, const void* t, const void* s, long lo
  * incident field; host out2 = (C_ext, C_abs). The reference has no counterpart. */
 int pdda_cross_sections(int prec, const void* P, const void* E_inc, const void* inv_alpha, long long nsites,
                         double k, double* out2, void* stream);
+
+/* Target setup. shapes_host: nshapes records of 16 doubles (type 0 sphere / 1
+ * ellipsoid / 2 rectangle, centre[3], radius | semi-axes[3] | dimensions[3],
+ * value re/im, site window lo/hi per axis, padding); later records overwrite earlier
+ * ones. Writes the complex128 value grid (nx,ny,nz). Replaces mark_shapes and the
+ * numba occupancy tests (ddaforgit.py:440-474, 615-661). */
+int pdda_mark_shapes(const double* shapes_host, int nshapes, int nx, int ny, int nz, double d, int voxel_center,
+                     void* out_c128, void* stream);
+/* Bounding box of the nonzero sites: host bbox[6] = (min x,y,z, max x,y,z); max < 0
+ * when the grid is empty. With pdda_crop_pad it replaces trim_zero_faces
+ * (ddaforgit.py:831-857). */
+int pdda_trim_bbox(const void* a_c128, int nx, int ny, int nz, int* bbox_host, void* stream);
+/* out (ox,oy,oz) = in[x0:x0+cx, y0:y0+cy, z0:z0+cz] zero-padded at the high end. */
+int pdda_crop_pad(const void* in_c128, int nx, int ny, int nz, int x0, int y0, int z0, int cx, int cy, int cz,
+                  int ox, int oy, int oz, void* out_c128, void* stream);
+/* nrhs plane waves E[r] = e_r exp(i k_r . (i,j,l) d), complex128 (nrhs,nx,ny,nz,3);
+ * dirs_host holds (k_r[3] already scaled by k, e_r[3]) per rhs (ddaforgit.py:1626-1632). */
+int pdda_plane_wave(const double* dirs_host, int nrhs, int nx, int ny, int nz, double d, void* out_c128,
+                    void* stream);
 
 /* Device-resident preconditioned GPBiCG for `nslots` right-hand sides sharing one box.
  * Replaces solve_gpbicgstab (Gpbicgstab.py:879-1201). precon=0 makes the

@@ -46,6 +46,7 @@ _SIGS = {
     "pdda_conv_destroy": (_i, [_vp]),
     "pdda_conv_workspace_bytes": (_ll, [_vp]),
     "pdda_conv_apply": (_i, [_vp, _i, _vp, _vp, _vp, _ll, _vp, _vp, _ll, _vp, _vp]),
+    "pdda_conv_time_passes": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _i, _c.POINTER(_d), _vp]),
     "pdda_dot": (_i, [_i, _vp, _vp, _ll, _c.POINTER(_d), _vp]),
     "pdda_zeta_eta": (_i, [_i, _vp, _vp, _vp, _ll, _c.POINTER(_d), _vp]),
     "pdda_cross_sections": (_i, [_i, _vp, _vp, _vp, _ll, _d, _c.POINTER(_d), _vp]),
@@ -61,6 +62,10 @@ _SIGS = {
     "pdda_mixed_bind": (_i, [_vp, _vp, _vp, _ll, _vp, _ll, _vp, _vp, _vp, _ll]),
     "pdda_mixed_solve": (_i, [_vp, _vp, _d, _d, _i, _i, _d, _i, _i, _vp]),
     "pdda_mixed_result": (_i, [_vp, _vp, _c.POINTER(MixedReport), _i, _vp]),
+    "pdda_mark_shapes": (_i, [_c.POINTER(_d), _i, _i, _i, _i, _d, _i, _vp, _vp]),
+    "pdda_trim_bbox": (_i, [_vp, _i, _i, _i, _c.POINTER(_i), _vp]),
+    "pdda_crop_pad": (_i, [_vp, _i, _i, _i, _i, _i, _i, _i, _i, _i, _i, _i, _i, _vp, _vp]),
+    "pdda_plane_wave": (_i, [_c.POINTER(_d), _i, _i, _i, _i, _d, _vp, _vp]),
 }
 
 EXPORTED_SYMBOLS = tuple(_SIGS.keys())

@@ -203,6 +203,20 @@ int pdda_conv_apply(pdda_conv* c, int mode, const void* x, void* out, const void
   return guarded([&] { return conv_apply(&c->p, a, (hipStream_t)stream); });
 }
 
+int pdda_conv_time_passes(pdda_conv* c, int mode, const void* x, void* out, const void* table,
+                          const void* inv_alpha, const void* mask, int reps, double* ms5, void* stream) {
+  if (!c) { set_error("null plan"); return PDDA_ERR_ARG; }
+  ConvArgs a;
+  a.mode = mode;
+  a.x = x;
+  a.out = out;
+  a.table = table;
+  a.inv_alpha = inv_alpha;
+  a.mask = (const uint8_t*)mask;
+  a.b = x;
+  return guarded([&] { return conv_time_passes(&c->p, a, reps, ms5, (hipStream_t)stream); });
+}
+
 int pdda_dot(int prec, const void* a, const void* b, long long n, double* out2, void* stream) {
   if (!a || !b || !out2 || n < 0) { set_error("pdda_dot: bad arguments"); return PDDA_ERR_ARG; }
   return prec ? run_reduction<double, 0>(a, b, nullptr, n, out2, (hipStream_t)stream)

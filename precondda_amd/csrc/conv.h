@@ -73,6 +73,9 @@ int make_fft_plan(int L, FftPlan* out);
 int conv_plan_init(ConvPlanBase* p, int prec, int nx, int ny, int nz, int hx, int hy, int hz, int nslots);
 void conv_plan_free(ConvPlanBase* p);
 int conv_apply(ConvPlanBase* p, const ConvArgs& a, hipStream_t s);
+// Runs `reps` convolutions and returns the mean duration of each pass P1..P5 in ms,
+// measured with HIP events recorded between the passes on stream s.
+int conv_time_passes(ConvPlanBase* p, const ConvArgs& a, int reps, double* ms5, hipStream_t s);
 int green_build(int nx, int ny, int nz, double k, double d, void* out64, void* out32, hipStream_t s);
 int precond_build(int hx, int hy, int hz, double gre, double gim, const void* g64, void* out64, void* out32,
                   hipStream_t s);

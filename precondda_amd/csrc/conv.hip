@@ -40,37 +40,34 @@ __device__ inline void finalize_dots(SlotCtl& c, int kind, const double* re, con
   }
 }
 
-// Workgroup partial -> per-slot partial array; the last workgroup of the slot (ticket)
-// sums all partials in a fixed order and finalizes the slot's scalars.
+// Workgroup partial sums -> partial[wg][ND] of the slot (plain stores; the finalize
+// kernel launched next on the same stream reads them after the kernel boundary).
 template <int ND>
-__device__ inline void grid_reduce(double (&re)[ND], double (&im)[ND], cd* partial, int nwg, int wg,
-                                   int* counter, SlotCtl* ctl, int kind) {
+__device__ inline void write_partial(double (&re)[ND], double (&im)[ND], cd* partial, int wg) {
   __shared__ double scratch[2 * ND * (FFT_NT / 64)];
-  __shared__ int is_last;
   block_sum<ND>(re, im, scratch);
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0)
     for (int d = 0; d < ND; ++d) partial[(size_t)wg * ND + d] = cd{re[d], im[d]};
-    __threadfence();
-    const int t = atomicAdd(counter, 1);
-    is_last = (t == nwg - 1);
-  }
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
-  const volatile double* pv = reinterpret_cast<const volatile double*>(partial);
-  double r2[ND], i2[ND];
-  for (int d = 0; d < ND; ++d) { r2[d] = 0.0; i2[d] = 0.0; }
-  for (int w = threadIdx.x; w < nwg; w += blockDim.x) {
+}
+
+// One workgroup per slot: sums the slot's partials in a fixed order and finalizes
+// the slot's Krylov scalars.
+template <int ND>
+__global__ __launch_bounds__(FFT_NT) void k_finalize_dots(const cd* __restrict__ partial, int nwg, int stride,
+                                                          SlotCtl* ctl, int kind, int need) {
+  __shared__ double scratch[2 * ND * (FFT_NT / 64)];
+  const int slot = blockIdx.x;
+  if (!slot_wanted(ctl, slot, need)) return;
+  const cd* p = partial + (size_t)slot * stride;
+  double re[ND], im[ND];
+  for (int d = 0; d < ND; ++d) { re[d] = 0.0; im[d] = 0.0; }
+  for (int w = threadIdx.x; w < nwg; w += FFT_NT)
     for (int d = 0; d < ND; ++d) {
-      r2[d] += pv[2 * ((size_t)w * ND + d)];
-      i2[d] += pv[2 * ((size_t)w * ND + d) + 1];
+      re[d] += p[(size_t)w * ND + d].x;
+      im[d] += p[(size_t)w * ND + d].y;
     }
-  }
-  block_sum<ND>(r2, i2, scratch);
-  if (threadIdx.x == 0) {
-    finalize_dots<ND>(*ctl, kind, r2, i2);
-    atomicExch(counter, 0);
-  }
+  block_sum<ND>(re, im, scratch);
+  if (threadIdx.x == 0) finalize_dots<ND>(ctl[slot], kind, re, im);
 }
 
 // ---------------------------------------------------------------------------------
@@ -300,9 +297,7 @@ __global__ __launch_bounds__(FFT_NT) void k_zinv(const C<T>* __restrict__ A, C<T
       sre[5] += tx * sx + ty2 * sy; sim[5] += tx * sy - ty2 * sx;  // <t,s>
     }
   }
-  if (DOTS != DOT_NONE) {
-    grid_reduce<ND>(sre, sim, partial + (size_t)slot * nwg * 6, nwg, wg, counters + slot, ctl + slot, DOTS);
-  }
+  if (DOTS != DOT_NONE) write_partial<ND>(sre, sim, partial + (size_t)slot * nwg * 6, wg);
 }
 
 // ---------------------------------------------------------------------------------
@@ -550,17 +545,31 @@ static void launch_p5(ConvPlanBase* p, const ConvArgs& a, hipStream_t s) {
 
 template <typename T, int MODE>
 static void launch_p5_dots(ConvPlanBase* p, const ConvArgs& a, hipStream_t s) {
+  const int nsl = p->g.nslots, stride = p->nwg5 * 6;
   switch (a.dots) {
-    case DOT_ALPHA: launch_p5<T, MODE, DOT_ALPHA>(p, a, s); break;
-    case DOT_ZETA: launch_p5<T, MODE, DOT_ZETA>(p, a, s); break;
-    case DOT_RESID: launch_p5<T, MODE, DOT_RESID>(p, a, s); break;
+    case DOT_ALPHA:
+      launch_p5<T, MODE, DOT_ALPHA>(p, a, s);
+      hipLaunchKernelGGL(k_finalize_dots<1>, dim3(nsl), dim3(FFT_NT), 0, s, (const cd*)p->partial, p->nwg5, stride,
+                         a.ctl, (int)DOT_ALPHA, a.need);
+      break;
+    case DOT_ZETA:
+      launch_p5<T, MODE, DOT_ZETA>(p, a, s);
+      hipLaunchKernelGGL(k_finalize_dots<6>, dim3(nsl), dim3(FFT_NT), 0, s, (const cd*)p->partial, p->nwg5, stride,
+                         a.ctl, (int)DOT_ZETA, a.need);
+      break;
+    case DOT_RESID:
+      launch_p5<T, MODE, DOT_RESID>(p, a, s);
+      hipLaunchKernelGGL(k_finalize_dots<2>, dim3(nsl), dim3(FFT_NT), 0, s, (const cd*)p->partial, p->nwg5, stride,
+                         a.ctl, (int)DOT_RESID, a.need);
+      break;
     default: launch_p5<T, MODE, DOT_NONE>(p, a, s); break;
   }
 }
 
 template <typename T>
-static int conv_apply_t(ConvPlanBase* p, const ConvArgs& a, hipStream_t s) {
+static int conv_apply_t(ConvPlanBase* p, const ConvArgs& a, hipStream_t s, hipEvent_t* ev = nullptr) {
   const ConvGeom& g = p->g;
+  if (ev) PDDA_HIP_CHECK(hipEventRecord(ev[0], s));
   C<T>* A = (C<T>*)p->work_a;
   C<T>* B = (C<T>*)p->work_b;
   // P1
@@ -573,6 +582,7 @@ static int conv_apply_t(ConvPlanBase* p, const ConvArgs& a, hipStream_t s) {
       hipLaunchKernelGGL((k_zfwd<T, false>), dim3(nblk), dim3(FFT_NT), p->lds[0], s, (const C<T>*)a.x, A, g, p->pz,
                          (const C<T>*)p->tw[2], a.ctl, a.need);
   }
+  if (ev) PDDA_HIP_CHECK(hipEventRecord(ev[1], s));
   // P2
   {
     AxisArgs ax;
@@ -587,6 +597,7 @@ static int conv_apply_t(ConvPlanBase* p, const ConvArgs& a, hipStream_t s) {
       hipLaunchKernelGGL((k_axis<T, false, false>), dim3((unsigned)nblk), dim3(FFT_NT), p->lds[1], s, (const C<T>*)A,
                          B, ax, p->py, (const C<T>*)p->tw[1], a.ctl, a.need);
   }
+  if (ev) PDDA_HIP_CHECK(hipEventRecord(ev[2], s));
   // P3
   {
     const int nblk = g.nslots * g.Ly * ceil_div(g.Lz, g.wz3);
@@ -598,6 +609,7 @@ static int conv_apply_t(ConvPlanBase* p, const ConvArgs& a, hipStream_t s) {
       hipLaunchKernelGGL((k_xconv<T, false>), dim3(nblk), dim3(FFT_NT), p->lds[2], s, B, (const C<T>*)a.table,
                          a.table_stride, g, p->px, (const C<T>*)p->tw[0], a.ctl, a.need, scale);
   }
+  if (ev) PDDA_HIP_CHECK(hipEventRecord(ev[3], s));
   // P4
   {
     AxisArgs ax;
@@ -612,6 +624,7 @@ static int conv_apply_t(ConvPlanBase* p, const ConvArgs& a, hipStream_t s) {
       hipLaunchKernelGGL((k_axis<T, true, false>), dim3((unsigned)nblk), dim3(FFT_NT), p->lds[3], s, (const C<T>*)B,
                          A, ax, p->py, (const C<T>*)p->tw[1], a.ctl, a.need);
   }
+  if (ev) PDDA_HIP_CHECK(hipEventRecord(ev[4], s));
   // P5
   switch (a.mode) {
     case MODE_FIELD: launch_p5_dots<T, MODE_FIELD>(p, a, s); break;
@@ -620,6 +633,7 @@ static int conv_apply_t(ConvPlanBase* p, const ConvArgs& a, hipStream_t s) {
     case MODE_RESIDUAL: launch_p5_dots<T, MODE_RESIDUAL>(p, a, s); break;
     default: set_error("unknown convolution mode"); return PDDA_ERR_ARG;
   }
+  if (ev) PDDA_HIP_CHECK(hipEventRecord(ev[5], s));
   PDDA_LAUNCH_CHECK();
   return PDDA_OK;
 }
@@ -635,6 +649,27 @@ int conv_apply(ConvPlanBase* p, const ConvArgs& a, hipStream_t s) {
     set_error("conv_apply: fused dots need ctl and operand vectors"); return PDDA_ERR_ARG;
   }
   return p->prec ? conv_apply_t<double>(p, a, s) : conv_apply_t<float>(p, a, s);
+}
+
+int conv_time_passes(ConvPlanBase* p, const ConvArgs& a, int reps, double* ms5, hipStream_t s) {
+  if (reps <= 0 || !ms5) { set_error("conv_time_passes: bad arguments"); return PDDA_ERR_ARG; }
+  hipEvent_t ev[6];
+  for (int i = 0; i < 6; ++i) PDDA_HIP_CHECK(hipEventCreate(&ev[i]));
+  for (int q = 0; q < 5; ++q) ms5[q] = 0.0;
+  int rc = PDDA_OK;
+  for (int r = 0; r < reps && rc == PDDA_OK; ++r) {
+    rc = p->prec ? conv_apply_t<double>(p, a, s, ev) : conv_apply_t<float>(p, a, s, ev);
+    if (rc) break;
+    if (hipEventSynchronize(ev[5]) != hipSuccess) { set_error("conv_time_passes: event sync failed"); rc = PDDA_ERR_HIP; break; }
+    for (int q = 0; q < 5; ++q) {
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, ev[q], ev[q + 1]);
+      ms5[q] += ms;
+    }
+  }
+  for (int q = 0; q < 5; ++q) ms5[q] /= reps;
+  for (int i = 0; i < 6; ++i) (void)hipEventDestroy(ev[i]);
+  return rc;
 }
 
 int green_build(int nx, int ny, int nz, double k, double d, void* out64, void* out32, hipStream_t s) {

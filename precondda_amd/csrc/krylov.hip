@@ -57,34 +57,32 @@ __device__ inline void finalize_vec(SlotCtl& c, int kind, const double* re, cons
 }
 
 template <int ND>
-__device__ inline void vec_grid_reduce(double (&re)[ND], double (&im)[ND], cd* partial, int nwg, int wg,
-                                       int* counter, SlotCtl* ctl, int kind, double p0 = 0, double p1 = 0,
-                                       int ip0 = 0, int ip1 = 0) {
+__device__ inline void vec_partial(double (&re)[ND], double (&im)[ND], cd* partial, int wg) {
   __shared__ double scratch[2 * ND * (VNT / 64)];
-  __shared__ int is_last;
   block_sum<ND>(re, im, scratch);
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0)
     for (int d = 0; d < ND; ++d) partial[(size_t)wg * ND + d] = cd{re[d], im[d]};
-    __threadfence();
-    const int t = atomicAdd(counter, 1);
-    is_last = (t == nwg - 1);
-  }
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
-  const volatile double* pv = reinterpret_cast<const volatile double*>(partial);
-  double r2[ND], i2[ND];
-  for (int d = 0; d < ND; ++d) { r2[d] = 0.0; i2[d] = 0.0; }
-  for (int w = threadIdx.x; w < nwg; w += blockDim.x)
+}
+
+// One workgroup per slot sums the partials of the preceding vector kernel in a fixed
+// order. only_active: skip slots without F_ACTIVE (the producer skipped them too).
+template <int ND>
+__global__ __launch_bounds__(VNT) void k_vec_finalize(const cd* __restrict__ partial, int nwg, SlotCtl* ctl,
+                                                      int slot0, int kind, int only_active, double p0, double p1,
+                                                      int ip0, int ip1) {
+  __shared__ double scratch[2 * ND * (VNT / 64)];
+  const int slot = slot0 + blockIdx.x;
+  if (only_active && !(ctl[slot].flags & F_ACTIVE)) return;
+  const cd* p = partial + (size_t)slot * nwg * 6;
+  double re[ND], im[ND];
+  for (int d = 0; d < ND; ++d) { re[d] = 0.0; im[d] = 0.0; }
+  for (int w = threadIdx.x; w < nwg; w += VNT)
     for (int d = 0; d < ND; ++d) {
-      r2[d] += pv[2 * ((size_t)w * ND + d)];
-      i2[d] += pv[2 * ((size_t)w * ND + d) + 1];
+      re[d] += p[(size_t)w * ND + d].x;
+      im[d] += p[(size_t)w * ND + d].y;
     }
-  block_sum<ND>(r2, i2, scratch);
-  if (threadIdx.x == 0) {
-    finalize_vec(*ctl, kind, r2, i2, p0, p1, ip0, ip1);
-    atomicExch(counter, 0);
-  }
+  block_sum<ND>(re, im, scratch);
+  if (threadIdx.x == 0) finalize_vec(ctl[slot], kind, re, im, p0, p1, ip0, ip1);
 }
 
 template <typename T> __device__ inline C<T> cz() { return mk<T>(0, 0); }
@@ -121,8 +119,7 @@ __global__ __launch_bounds__(VNT) void k_init(const C<T>* __restrict__ E, const 
     v.s1[off + i] = cz<T>();
     re[0] += (double)b.x * b.x + (double)b.y * b.y;
   }
-  vec_grid_reduce<1>(re, im, partial + (size_t)slot * nwg * 6, nwg, blockIdx.x, counters + slot, ctl + slot,
-                     FIN_INIT, target, ratio, max_iter, stagnation);
+  vec_partial<1>(re, im, partial + (size_t)slot * nwg * 6, blockIdx.x);
 }
 
 // s = r - alpha v ; y = s_prev - s - alpha w
@@ -180,8 +177,7 @@ __global__ __launch_bounds__(VNT) void k_upd2(VecSet<T> v, const C<T>* __restric
     re[1] += (double)h.x * rn.x + (double)h.y * rn.y;
     im[1] += (double)h.x * rn.y - (double)h.y * rn.x;
   }
-  vec_grid_reduce<2>(re, im, partial + (size_t)slot * nwg * 6, nwg, blockIdx.x, counters + slot, ctl + slot,
-                     FIN_STORE);
+  vec_partial<2>(re, im, partial + (size_t)slot * nwg * 6, blockIdx.x);
 }
 
 // current_norm = sqrt(<r,r>) for the slots whose residual was just recomputed.
@@ -342,8 +338,7 @@ __global__ __launch_bounds__(VNT) void k_refine_init(const double2* __restrict__
     x[off + i] = make_double2(0.0, 0.0);
     re[0] += bv.x * bv.x + bv.y * bv.y;
   }
-  vec_grid_reduce<1>(re, im, partial + (size_t)slot * nwg * 6, nwg, blockIdx.x, counters + slot, ctl64 + slot,
-                     FIN_NORM);
+  vec_partial<1>(re, im, partial + (size_t)slot * nwg * 6, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------------
@@ -400,6 +395,8 @@ int SolverT<T>::load(int slot, const void* E, double target, int max_iter, doubl
   if (s0 + ns > nslots) { set_error("solver: slot out of range"); return PDDA_ERR_ARG; }
   hipLaunchKernelGGL(k_init<T>, dim3(nwg, ns), dim3(VNT), 0, s, (const C<T>*)E, mask, site_stride, vs, n3, s0,
                      partial, counters, ctl, target, ratio, max_iter, stagnation, nwg);
+  hipLaunchKernelGGL(k_vec_finalize<1>, dim3(ns), dim3(VNT), 0, s, (const cd*)partial, nwg, ctl, s0, (int)FIN_INIT, 0,
+                     target, ratio, max_iter, stagnation);
   sidx = 0;
   host_iter = 0;
   PDDA_LAUNCH_CHECK();
@@ -452,6 +449,8 @@ int SolverT<T>::iterate(int n_iter, hipStream_t s) {
     if ((rc = conv(&mv, MODE_INTERACT, sh, vs.t, ghat, ghat_stride, DOT_ZETA, vs.y, s_new, NEED_ACTIVE, s)))
       return rc;
     hipLaunchKernelGGL(k_upd2<T>, vg, dim3(VNT), 0, s, vs, s_new, s_prev, vs.y, n3, partial, counters, ctl, nwg);
+    hipLaunchKernelGGL(k_vec_finalize<2>, dim3(nslots), dim3(VNT), 0, s, (const cd*)partial, nwg, ctl, 0,
+                       (int)FIN_STORE, 1, 0.0, 0.0, 0, 0);
     // true-residual replacement at the slot's iteration 1
     if (!lockstep || host_iter == 1) {
       const C<T>* xs = vs.x;
@@ -570,6 +569,8 @@ int MixedSolver::solve(const void* E64, double tol, double inner_target, int max
   const dim3 vg(nwg, nslots);
   hipLaunchKernelGGL(k_refine_init, vg, dim3(VNT), 0, s, (const double2*)E64, inner.mask, inner.site_stride, b64,
                      x64, r64, n3, partial, counters, ctl64, nwg);
+  hipLaunchKernelGGL(k_vec_finalize<1>, dim3(nslots), dim3(VNT), 0, s, (const cd*)partial, nwg, ctl64, 0,
+                     (int)FIN_NORM, 0, 0.0, 0.0, 0, 0);
   PDDA_LAUNCH_CHECK();
   PDDA_HIP_CHECK(hipMemcpyAsync(ctl64_host, ctl64, nslots * sizeof(SlotCtl), hipMemcpyDeviceToHost, s));
   PDDA_HIP_CHECK(hipStreamSynchronize(s));

@@ -84,6 +84,14 @@ class ConvPlan:
             _lib._LIB.pdda_conv_destroy(h)
             self._h = None
 
+    def time_passes(self, mode: int, x, out, table, inv_alpha=None, mask=None, reps: int = 20, stream=None):
+        """Mean milliseconds of passes P1..P5 over `reps` launches (HIP events)."""
+        ptr = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None
+        ms = (ctypes.c_double * 5)()
+        check(lib().pdda_conv_time_passes(self._h, mode, ptr(x), ptr(out), ptr(table), ptr(inv_alpha), ptr(mask),
+                                          int(reps), ms, stream_ptr(stream)))
+        return list(ms)
+
     def workspace_bytes(self) -> int:
         return int(lib().pdda_conv_workspace_bytes(self._h))
 
@@ -141,20 +149,28 @@ def generate_interaction_row(nx, ny, nz, k, lattice_spacing, reduced=True, doubl
     return g64 if double_precision else g32
 
 
-def build_preconditioner(nx, ny, nz, k, lattice_spacing, inv_alpha, refract_mult=1.0, expansion=None,
+# Complex multiplier used when none is given: it damps the periodic medium the circulant
+# inverts. Chosen by a sweep over spheres and cubes with |m| kd <= 0.5 (the undamped
+# multiplier 1 makes GPBiCG diverge from x ~ 10 on).
+DEFAULT_REFRACT_MULT = 1.0 - 2.0j
+
+
+def build_preconditioner(nx, ny, nz, k, lattice_spacing, inv_alpha, refract_mult=None, expansion=None,
                          both=False, double_precision=False):
     """Circulant preconditioner table M(q) = (gamma I - Ghat_P(q))^-1, reduced format.
 
     Stands in for pre_preconditioner + circulant_approximation (ddaforgit.py:1118-1136),
     whose source is absent from the reference repository; applied by fft_matvec3
-    (Gpbicgstab.py:633-668). gamma = refract_mult * mean(inv_alpha over occupied sites);
-    the table lives on the (2Px, 2Py, 2Pz) grid, P = expansion (default: the box).
+    (Gpbicgstab.py:633-668). gamma = refract_mult * mean(inv_alpha over occupied sites),
+    refract_mult being the reference's complex multiplier (ddaforgit.py:1134; there
+    predicted by a learned model) or DEFAULT_REFRACT_MULT when None. The table lives on
+    the (2Px, 2Py, 2Pz) grid, P = expansion (default: the box).
     """
     P = tuple(int(v) for v in (expansion if expansion is not None else (nx, ny, nz)))
     ia = as_device(inv_alpha, C128).reshape(-1)
     occ = ia != 0
     gam = complex(ia[occ].mean().item()) if bool(occ.any()) else 1.0 + 0j
-    gam *= complex(refract_mult)
+    gam *= complex(DEFAULT_REFRACT_MULT if refract_mult is None else refract_mult)
     g64 = generate_interaction_row(*P, k, lattice_spacing, double_precision=True)
     m64 = torch.empty_like(g64)
     m32 = torch.empty(g64.shape, dtype=C64, device=g64.device) if (both or not double_precision) else None

@@ -1,0 +1,254 @@
+"""Target and lattice setup: shapes, polarisability, occupancy marking, trimming and
+the incident plane wave (ddaforgit.py:54-437, 615-661, 812-857, 1591-1645).
+
+Shape bookkeeping (ids, centre snapping, one polarisability per refractive index) is
+host Python like the reference's; the per-site work (occupancy tests over the grid,
+bounding box, crop-and-pad, plane-wave phases) runs in libpdda.so on the device.
+Supported shapes on the device: sphere, ellipsoid, rectangle (the shapes of the solve
+workloads); cylinders and prisms raise ValueError.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from typing import Dict, List, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, lib, stream_ptr
+
+LDR_B1, LDR_B2, LDR_B3 = -1.891531, 0.1648469, -1.7700004  # Draine & Goodman 1993 (ddaforgit.py:428-430)
+_EDGE_EPS = 1e-7
+
+
+def calculate_alpha(refractive_index, lattice_spacing, wave_number, incident_direction=None,
+                    polarization_direction=None, method="LDR"):
+    """Polarisability per dipole: CM, RR or LDR (ddaforgit.py:403-437).
+
+    The LDR term uses the reference's S = (a.e)^2 (ddaforgit.py:432).
+    """
+    n = complex(refractive_index)
+    d = float(lattice_spacing)
+    kd = float(wave_number) * d
+    eps = n * n
+    a_cm = (3.0 * d ** 3 / (4.0 * math.pi)) * (eps - 1.0) / (eps + 2.0)
+    if method == "CM":
+        return np.complex128(a_cm)
+    rr = (2.0 / 3.0) * 1j * kd ** 3
+    if method == "RR":
+        return np.complex128(a_cm / (1.0 + (a_cm / d ** 3) * (-rr)))
+    if method == "LDR":
+        if incident_direction is None or polarization_direction is None:
+            raise ValueError("incident_direction and polarization_direction required for LDR method")
+        S = float(np.dot(np.asarray(incident_direction, float), np.asarray(polarization_direction, float))) ** 2
+        ldr = (LDR_B1 + eps * LDR_B2 + eps * LDR_B3 * S) * kd ** 2
+        return np.complex128(a_cm / (1.0 + (a_cm / d ** 3) * (ldr - rr)))
+    raise ValueError("method must be 'CM', 'RR', or 'LDR'")
+
+
+# ---------------------------------------------------------------------------------
+# Shapes (ddaforgit.py:54-106) and ShapeManager (ddaforgit.py:109-400)
+# ---------------------------------------------------------------------------------
+class Shape:
+    def __init__(self, shape_id, n, group=None, priority=0):
+        self.id = shape_id
+        self.n = n
+        self.group = group
+        self.priority = priority
+        self.value = None
+        self.center = [0, 0, 0]
+
+
+class Sphere(Shape):
+    def __init__(self, shape_id, center, radius, n, group=None, priority=0):
+        super().__init__(shape_id, n, group, priority)
+        self.center = center
+        self.radius = radius
+
+
+class Ellipsoid(Shape):
+    def __init__(self, shape_id, center, semi_axes, n, axis="z", group=None, priority=0):
+        super().__init__(shape_id, n, group, priority)
+        self.center = center
+        self.semi_axes = semi_axes
+        self.axis = axis
+
+
+class Rectangle(Shape):
+    def __init__(self, shape_id, center, dimensions, n, group=None, priority=0):
+        super().__init__(shape_id, n, group, priority)
+        self.center = center
+        self.dimensions = dimensions
+
+
+_SHAPE_CLASSES = {"sphere": Sphere, "ellipsoid": Ellipsoid, "rectangle": Rectangle}
+_TYPE_CODE = {Sphere: 0, Ellipsoid: 1, Rectangle: 2}
+
+
+def _max_extent(s) -> float:
+    return max(float(getattr(s, "radius", 0)), float(getattr(s, "height", 0)) / 2,
+               float(max(getattr(s, "semi_axes", [0, 0, 0]))),
+               float(max(getattr(s, "dimensions", [0, 0, 0]))) / 2)
+
+
+class ShapeManager:
+    """Shape registry with the reference's id scheme and processing rules."""
+
+    def __init__(self):
+        self.shapes: Dict[str, Shape] = {}
+        self.shape_counters: Dict[str, int] = {}
+
+    def get_next_id(self, shape_type):
+        self.shape_counters[shape_type] = self.shape_counters.get(shape_type, 0) + 1
+        return f"{shape_type}_{self.shape_counters[shape_type]}"
+
+    def add_shape(self, shape_type, **params):
+        cls = _SHAPE_CLASSES.get(shape_type)
+        if cls is None:
+            raise ValueError(f"Unknown shape type: {shape_type}")
+        shape_id = params.pop("force_id", None) or self.get_next_id(shape_type)
+        self.shapes[shape_id] = cls(shape_id, **params)
+        return shape_id
+
+    def remove_shape(self, shape_id):
+        if shape_id not in self.shapes:
+            raise ValueError(f"Shape ID {shape_id} not found")
+        del self.shapes[shape_id]
+
+    def process_shapes(self, k, E_direction, E_polarization, lattice_spacing, adjust_all=True):
+        """One LDR alpha per distinct n; centres snapped up with ceil (ddaforgit.py:291-343)."""
+        if not self.shapes:
+            return
+        ordered = sorted(self.shapes.values(), key=lambda s: s.priority)
+        amap = {}
+        for s in ordered:
+            if s.n not in amap:
+                amap[s.n] = calculate_alpha(s.n, lattice_spacing, k, E_direction, E_polarization, method="LDR")
+            s.value = amap[s.n]
+        d = float(lattice_spacing)
+        centers = np.array([s.center for s in ordered], dtype=np.float64)
+        if len(ordered) == 1 or adjust_all:
+            adj = d * np.ceil(centers / d)
+        else:
+            big = max(ordered, key=_max_extent)
+            base = np.array(big.center, dtype=np.float64)
+            adj = centers + (d * np.ceil(base / d) - base)
+        for s, c in zip(ordered, adj):
+            s.center = c.tolist()
+
+    def calculate_grid_extents(self, lattice_spacing, max_x=1e9, max_y=1e9, max_z=1e9):
+        """Box covering every shape plus one spacing (ddaforgit.py:346-400)."""
+        if not self.shapes:
+            return 0, 0, 0
+        mx = np.zeros(3)
+        for s in self.shapes.values():
+            mx = np.maximum(mx, np.asarray(s.center, dtype=np.float64) + _max_extent(s))
+        mx = np.minimum(mx, [max_x, max_y, max_z]) + lattice_spacing
+        lim = [max_x, max_y, max_z]
+        return tuple(min(int(np.ceil(mx[a] / lattice_spacing)), int(np.ceil(lim[a] / lattice_spacing)))
+                     for a in range(3))
+
+
+def _shape_records(shapes: Sequence[Shape], grid, d: float, field: str) -> np.ndarray:
+    recs = np.zeros((len(shapes), 16), dtype=np.float64)
+    for q, s in enumerate(shapes):
+        t = _TYPE_CODE.get(type(s))
+        if t is None:
+            raise ValueError(f"shape {type(s).__name__} is not supported on the device")
+        c = np.asarray(s.center, dtype=np.float64)
+        recs[q, 0] = t
+        recs[q, 1:4] = c
+        if t == 0:
+            recs[q, 4] = float(s.radius)
+        elif t == 1:
+            recs[q, 4:7] = np.asarray(s.semi_axes, dtype=np.float64)
+        else:
+            recs[q, 4:7] = np.asarray(s.dimensions, dtype=np.float64)
+        v = complex(getattr(s, field))
+        recs[q, 7], recs[q, 8] = v.real, v.imag
+        # marking window of ddaforgit.py:626-636
+        mgd = int(float(_max_extent(s) + _EDGE_EPS) / d) + 1
+        ci = (c / d).astype(int)
+        for a in range(3):
+            recs[q, 9 + 2 * a] = max(0, ci[a] - mgd)
+            recs[q, 10 + 2 * a] = min(grid[a], ci[a] + mgd + 1)
+    return recs
+
+
+def mark_shapes(grid_shape, lattice_spacing, shapes, voxel_center=True, field="value") -> torch.Tensor:
+    """Value grid (nx,ny,nz) complex128 on the device (ddaforgit.py:615-661)."""
+    _lib.require_gpu()
+    grid = tuple(int(v) for v in grid_shape)
+    shapes = list(shapes)
+    recs = np.ascontiguousarray(_shape_records(shapes, grid, float(lattice_spacing), field))
+    out = torch.empty(grid, dtype=torch.complex128, device="cuda")
+    check(lib().pdda_mark_shapes(recs.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(shapes), *grid,
+                                 float(lattice_spacing), 1 if voxel_center else 0, ctypes.c_void_p(out.data_ptr()),
+                                 stream_ptr()))
+    return out
+
+
+def trim_zero_faces(array) -> torch.Tensor:
+    """Crop empty faces, pad odd dims to even at the high end (ddaforgit.py:831-857)."""
+    a = array if isinstance(array, torch.Tensor) else torch.as_tensor(np.asarray(array))
+    a = a.to(device="cuda", dtype=torch.complex128).contiguous()
+    nx, ny, nz = a.shape
+    bb = (ctypes.c_int * 6)()
+    check(lib().pdda_trim_bbox(ctypes.c_void_p(a.data_ptr()), nx, ny, nz, bb, stream_ptr()))
+    if bb[3] < 0:
+        return a[:0, :0, :0]
+    c = [bb[3 + q] - bb[q] + 1 for q in range(3)]
+    o = [v + (v % 2) for v in c]
+    out = torch.empty(o, dtype=torch.complex128, device=a.device)
+    check(lib().pdda_crop_pad(ctypes.c_void_p(a.data_ptr()), nx, ny, nz, bb[0], bb[1], bb[2], *c, *o,
+                              ctypes.c_void_p(out.data_ptr()), stream_ptr()))
+    return out
+
+
+def create_alpha_and_n_arrays(shape_manager: ShapeManager, lattice_spacing, voxel_center=True):
+    """(alpha_array, n_array, shape) after marking and trimming (ddaforgit.py:812-830)."""
+    grid = shape_manager.calculate_grid_extents(lattice_spacing)
+    shapes = list(shape_manager.shapes.values())
+    alpha = trim_zero_faces(mark_shapes(grid, lattice_spacing, shapes, voxel_center, "value"))
+    n_arr = trim_zero_faces(mark_shapes(grid, lattice_spacing, shapes, voxel_center, "n"))
+    return alpha, n_arr, tuple(alpha.shape)
+
+
+def plane_waves(grid_size, k, directions, polarizations, lattice_spacing=1.0) -> torch.Tensor:
+    """(nrhs, nx, ny, nz, 3) complex128 plane waves e exp(i k a.(i,j,l) d) (ddaforgit.py:1626-1632)."""
+    _lib.require_gpu()
+    kd = np.asarray(directions, dtype=np.float64).reshape(-1, 3)
+    ep = np.asarray(polarizations, dtype=np.float64).reshape(-1, 3)
+    if kd.shape != ep.shape:
+        raise ValueError("directions and polarizations must pair up")
+    dirs = np.ascontiguousarray(np.concatenate([k * kd, ep], axis=1))
+    nx, ny, nz = (int(v) for v in grid_size)
+    out = torch.empty((len(kd), nx, ny, nz, 3), dtype=torch.complex128, device="cuda")
+    check(lib().pdda_plane_wave(dirs.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(kd), nx, ny, nz,
+                                float(lattice_spacing), ctypes.c_void_p(out.data_ptr()), stream_ptr()))
+    return out
+
+
+def build_problem_from_shapes(shapes: List[dict], k_direction, E_polarization, wavelength, lattice_spacing):
+    """Problem dict from shape specs (ddaforgit.py:1591-1651): alpha grid, grid size, E_inc, k."""
+    sm = ShapeManager()
+    for s in shapes:
+        t = s["type"]
+        if t == "sphere":
+            sm.add_shape("sphere", center=s["center"], radius=s["radius"], n=s["n"])
+        elif t == "rectangle":
+            sm.add_shape("rectangle", center=s["center"], dimensions=s["dimensions"], n=s["n"])
+        elif t == "ellipsoid":
+            sm.add_shape("ellipsoid", center=s["center"], semi_axes=s["semi_axes"], n=s["n"])
+        else:
+            raise ValueError(f"shape type {t} is not supported")
+    k = 2 * np.pi / wavelength
+    kdir = np.asarray(k_direction, dtype=np.float64)
+    sm.process_shapes(k, kdir, E_polarization, lattice_spacing)
+    alpha, n_arr, grid = create_alpha_and_n_arrays(sm, lattice_spacing)
+    E_inc = plane_waves(grid, k, kdir, E_polarization, lattice_spacing)[0]
+    return {"alpha_array": alpha, "n_array": n_arr, "grid_size": grid, "E_inc": E_inc, "k": k,
+            "lattice_spacing": lattice_spacing, "k_vec": k * kdir, "k_direction": kdir,
+            "E_polarization": np.asarray(E_polarization, dtype=np.float64), "wavelength": wavelength}

@@ -1,0 +1,118 @@
+"""The solve workloads: targets, wave numbers and incident fields of the five benchmark
+configurations, built on the device with the target-setup kernels.
+
+Conventions: lattice spacing d = 1, size parameter x = k a_eff with
+a_eff = (3 N_dipoles / 4 pi)^(1/3) d, LDR polarisability with the reference's S
+(direction independent), unit-amplitude plane waves.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+import torch
+
+from . import targets as T
+
+
+@dataclass
+class Workload:
+    name: str
+    grid: tuple
+    k: float
+    m: complex
+    alpha: torch.Tensor            # (nx,ny,nz) complex128 on the device
+    E: torch.Tensor                # (nrhs,nx,ny,nz,3) complex128 on the device
+    n_dipoles: int
+    directions: np.ndarray = field(default_factory=lambda: np.zeros((0, 3)))
+    polarizations: np.ndarray = field(default_factory=lambda: np.zeros((0, 3)))
+    description: str = ""
+
+
+def a_eff(n_dipoles: int, d: float = 1.0) -> float:
+    return (3.0 * n_dipoles / (4.0 * math.pi)) ** (1.0 / 3.0) * d
+
+
+def _count_dipoles(shapes: List[dict], d: float) -> tuple:
+    """Occupied-site count and trimmed box of a shape list (value = 1 everywhere)."""
+    sm = T.ShapeManager()
+    for s in shapes:
+        params = {k: v for k, v in s.items() if k != "type"}
+        params["n"] = 2.0
+        sm.add_shape(s["type"], **params)
+    for s in sm.shapes.values():
+        s.value = 1.0
+        s.center = list(d * np.ceil(np.asarray(s.center, dtype=np.float64) / d))
+    grid = sm.calculate_grid_extents(d)
+    occ = T.trim_zero_faces(T.mark_shapes(grid, d, sm.shapes.values(), True, "value"))
+    return int((occ != 0).sum().item()), tuple(occ.shape)
+
+
+def shape_workload(name: str, shapes: List[dict], m: complex, x: float, directions, polarizations, d=1.0,
+                   description="") -> Workload:
+    nd, _ = _count_dipoles(shapes, d)
+    k = x / a_eff(nd, d)
+    spec = [dict(s, n=m) for s in shapes]
+    pb = T.build_problem_from_shapes(spec, np.asarray(directions[0], float), np.asarray(polarizations[0], float),
+                                     2 * math.pi / k, d)
+    grid = pb["grid_size"]
+    E = T.plane_waves(grid, k, directions, polarizations, d)
+    return Workload(name, grid, k, m, pb["alpha_array"], E, nd, np.asarray(directions, float),
+                    np.asarray(polarizations, float), description)
+
+
+def sphere_small() -> Workload:
+    """Plain method, small: sphere R=8d on 16^3 (2176 dipoles), m=1.5+0.01i, x=2, +z, x-pol."""
+    return shape_workload("plain-small", [{"type": "sphere", "center": [8, 8, 8], "radius": 8}], 1.5 + 0.01j, 2.0,
+                          [[0, 0, 1]], [[1, 0, 0]],
+                          description="sphere R=8d on 16^3 (2176 dipoles), m=1.5+0.01i, x=2, +z, x-pol")
+
+
+def precision() -> Workload:
+    """Precision: sphere R=32d on 64^3 (137376 dipoles), m=1.33+0.001i, x=10, +z, x- and y-pol."""
+    return shape_workload("precision", [{"type": "sphere", "center": [32, 32, 32], "radius": 32}], 1.33 + 0.001j,
+                          10.0, [[0, 0, 1], [0, 0, 1]], [[1, 0, 0], [0, 1, 0]],
+                          description="sphere R=32d on 64^3 (137376 dipoles), m=1.33+0.001i, x=10, +z, "
+                                      "x- and y-polarised (2 RHS)")
+
+
+def batch_directions(n: int = 64, seed: int = 0):
+    """n directions uniform on the sphere from default_rng(seed), two orthogonal
+    polarisations each: e1 = normalize(z x k) (x x k if degenerate), e2 = k x e1."""
+    rng = np.random.default_rng(seed)
+    v = rng.standard_normal((n, 3))
+    kh = v / np.linalg.norm(v, axis=1, keepdims=True)
+    dirs, pols = [], []
+    for kk in kh:
+        e1 = np.cross([0.0, 0.0, 1.0], kk)
+        if np.linalg.norm(e1) < 1e-6:
+            e1 = np.cross([1.0, 0.0, 0.0], kk)
+        e1 /= np.linalg.norm(e1)
+        e2 = np.cross(kk, e1)
+        dirs += [kk, kk]
+        pols += [e1, e2]
+    return np.array(dirs), np.array(pols)
+
+
+def batching(n_dirs: int = 64) -> Workload:
+    """Batching: spheroid (24,24,48)d on 48x48x96 (115776 dipoles), m=1.6+0.05i, x=8, 128 RHS."""
+    dirs, pols = batch_directions(n_dirs, 0)
+    return shape_workload("batching", [{"type": "ellipsoid", "center": [24, 24, 48], "semi_axes": [24, 24, 48]}],
+                          1.6 + 0.05j, 8.0, dirs, pols,
+                          description=f"prolate spheroid (24,24,48)d on 48x48x96 (115776 dipoles), m=1.6+0.05i, "
+                                      f"x=8, {n_dirs} directions x 2 polarisations")
+
+
+def porous_cuboid(shape=(256, 256, 128), p=0.7, m=1.7 + 0.1j, x=30.0, seed=0) -> Workload:
+    """Large: porous cuboid, seeded Bernoulli occupancy p, +z, x- and y-pol."""
+    occ = np.random.default_rng(seed).random(shape) < p
+    nd = int(occ.sum())
+    k = x / a_eff(nd)
+    alpha_v = T.calculate_alpha(m, 1.0, k, [0, 0, 1], [1, 0, 0])
+    alpha = torch.as_tensor(occ).to("cuda").to(torch.complex128) * complex(alpha_v)
+    E = T.plane_waves(shape, k, [[0, 0, 1], [0, 0, 1]], [[1, 0, 0], [0, 1, 0]])
+    return Workload("large", tuple(shape), k, m, alpha, E, nd, np.array([[0, 0, 1.0]] * 2),
+                    np.array([[1.0, 0, 0], [0, 1.0, 0]]),
+                    f"porous cuboid {shape} p={p} (seed {seed}, {nd} dipoles), m={m}, x={x}")

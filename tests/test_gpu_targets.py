@@ -1,0 +1,51 @@
+"""Device target setup (marking, trimming, plane waves) against the oracle."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import dda_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_alpha(shapes, kdir, epol, wavelength):
+    return O.build_problem(shapes, kdir, epol, wavelength, 1.0)
+
+
+@pytest.mark.parametrize("case", [
+    [{"type": "sphere", "center": [8, 8, 8], "radius": 8, "n": 1.5 + 0.01j}],
+    [{"type": "ellipsoid", "center": [24, 24, 48], "semi_axes": [24, 24, 48], "n": 1.6 + 0.05j}],
+    [{"type": "rectangle", "center": [5.3, 4.0, 6.2], "dimensions": [7.0, 5.0, 9.0], "n": 1.7 + 0.1j}],
+    [{"type": "sphere", "center": [6.4, 5.5, 7.1], "radius": 5.2, "n": 1.4},
+     {"type": "rectangle", "center": [8.0, 8.0, 8.0], "dimensions": [4.0, 6.0, 3.0], "n": 2.0 + 0.2j},
+     {"type": "ellipsoid", "center": [5.0, 9.0, 6.0], "semi_axes": [2.5, 3.0, 4.5], "n": 1.3}],
+])
+def test_build_problem_matches_oracle(case):
+    from precondda_amd import targets as T
+    kdir, epol, wl = np.array([0.0, 0.0, 1.0]), np.array([1.0, 0.0, 0.0]), 25.0
+    ref = _oracle_alpha(case, kdir, epol, wl)
+    got = T.build_problem_from_shapes(case, kdir, epol, wl, 1.0)
+    assert tuple(got["grid_size"]) == tuple(ref["grid_size"])
+    a = got["alpha_array"].cpu().numpy()
+    assert np.array_equal(a != 0, ref["alpha_array"] != 0)
+    assert np.abs(a - ref["alpha_array"]).max() <= 1e-15 * np.abs(ref["alpha_array"]).max()
+    E = got["E_inc"].cpu().numpy()
+    assert np.abs(E - ref["E_inc"]).max() < 1e-13
+
+
+def test_plane_waves_oblique():
+    from precondda_amd import targets as T
+    from precondda_amd.workloads import batch_directions
+    dirs, pols = batch_directions(4, 0)
+    E = T.plane_waves((6, 5, 8), 0.37, dirs, pols).cpu().numpy()
+    for r in range(len(dirs)):
+        ref = O.plane_wave((6, 5, 8), 0.37, dirs[r], pols[r])
+        assert np.abs(E[r] - ref).max() < 1e-13
+        assert abs(np.dot(dirs[r], pols[r])) < 1e-12
+
+
+def test_workload_dipole_counts():
+    from precondda_amd import workloads as W
+    w = W.sphere_small()
+    assert w.grid == (16, 16, 16) and w.n_dipoles == 2176
+    assert abs(w.k - 0.24879399954352702) < 1e-12
