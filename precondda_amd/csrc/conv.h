@@ -36,6 +36,16 @@ struct ConvGeom {
   long long b_stride;    // per slot in B (nx*Ly*Lz*3)
 };
 
+// Strided-axis pass geometry: input viewed as (outer, in_rows, W); rows < n_load are
+// read (the rest are zeros), rows < n_store are written to (outer, out_rows, W).
+struct AxisArgs {
+  int n_outer, outer_per_slot;
+  int in_rows, n_load, out_rows, n_store;
+  long long W;
+  int ncol;
+  int negate;
+};
+
 struct ConvPlanBase {
   int prec;  // 0 = complex64, 1 = complex128
   ConvGeom g;
@@ -47,6 +57,12 @@ struct ConvPlanBase {
   int* counters = nullptr; // [nslots]
   int nwg5 = 0;            // P5 workgroups per slot
   size_t lds[5] = {0, 0, 0, 0, 0};
+  // compile-time-planned kernels (fast path) per pass and their tiles
+  int fast[5] = {0, 0, 0, 0, 0};
+  int f_rows = 0;   // P1/P5 rows per tile (power of 2)
+  int f_ncol = 0;   // P2/P4 columns per tile (power of 2)
+  int f_wz = 0;     // P3 kz per tile (power of 2)
+  size_t f_lds[5] = {0, 0, 0, 0, 0};
 };
 
 // Arguments of one convolution launch sequence.
@@ -65,7 +81,13 @@ struct ConvArgs {
   const void* d1 = nullptr;  // s (ZETA)
   SlotCtl* ctl = nullptr;
   int need = NEED_ALL;
+  bool fin_control = false;  // DOT_RESID finalize also runs the slot control step
 };
+
+// Fast path (conv_fast.hip): chooses compile-time-planned kernels and their tiles for
+// the passes whose FFT length has a compile-time plan; launches one pass.
+void fast_setup(ConvPlanBase* p);
+int fast_launch(ConvPlanBase* p, int pass, const ConvArgs& a, hipStream_t s);
 
 // Host API (conv.hip).
 bool fft_length_supported(int L);

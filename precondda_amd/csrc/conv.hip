@@ -54,7 +54,7 @@ __device__ inline void write_partial(double (&re)[ND], double (&im)[ND], cd* par
 // the slot's Krylov scalars.
 template <int ND>
 __global__ __launch_bounds__(FFT_NT) void k_finalize_dots(const cd* __restrict__ partial, int nwg, int stride,
-                                                          SlotCtl* ctl, int kind, int need) {
+                                                          SlotCtl* ctl, int kind, int need, int control) {
   __shared__ double scratch[2 * ND * (FFT_NT / 64)];
   const int slot = blockIdx.x;
   if (!slot_wanted(ctl, slot, need)) return;
@@ -67,7 +67,10 @@ __global__ __launch_bounds__(FFT_NT) void k_finalize_dots(const cd* __restrict__
       im[d] += p[(size_t)w * ND + d].y;
     }
   block_sum<ND>(re, im, scratch);
-  if (threadIdx.x == 0) finalize_dots<ND>(ctl[slot], kind, re, im);
+  if (threadIdx.x == 0) {
+    finalize_dots<ND>(ctl[slot], kind, re, im);
+    if (control) control_step(ctl[slot]);
+  }
 }
 
 // ---------------------------------------------------------------------------------
@@ -90,8 +93,9 @@ __global__ __launch_bounds__(FFT_NT) void k_zfwd(const C<T>* __restrict__ x, C<T
   load_twiddles<T>(tw, twz, g.Lz);
   const C<T>* src = x + slot * g.vec_stride + ((long long)ix * g.ny + y0) * g.nz * 3;
   const int rl = 3 * g.nz, nin = rows * rl;
+  const FastDiv drl = FastDiv::make((unsigned)rl);
   for (int e = threadIdx.x; e < nin; e += FFT_NT) {
-    const int row = e / rl, r2 = e - row * rl, p = r2 / 3, c = r2 - 3 * p;
+    const int row = (int)drl.div((unsigned)e), r2 = e - row * rl, p = r2 / 3, c = r2 - 3 * p;
     data[p * ncol + row * 3 + c] = src[e];
   }
   const int nzero = (g.Lz - g.nz) * ncol;
@@ -100,8 +104,9 @@ __global__ __launch_bounds__(FFT_NT) void k_zfwd(const C<T>* __restrict__ x, C<T
   tile_fft<T, false, GEN>(data, ncol, pz, tw);
   C<T>* dst = A + slot * g.a_stride + ((long long)ix * g.ny + y0) * g.Lz * 3;
   const int ro = 3 * g.Lz, nout = rows * ro;
+  const FastDiv dro = FastDiv::make((unsigned)ro);
   for (int e = threadIdx.x; e < nout; e += FFT_NT) {
-    const int row = e / ro, r2 = e - row * ro, p = r2 / 3, c = r2 - 3 * p;
+    const int row = (int)dro.div((unsigned)e), r2 = e - row * ro, p = r2 / 3, c = r2 - 3 * p;
     dst[e] = data[p * ncol + row * 3 + c];
   }
 }
@@ -111,14 +116,6 @@ __global__ __launch_bounds__(FFT_NT) void k_zfwd(const C<T>* __restrict__ x, C<T
 // Loads rows < n_load (zero to L), stores rows < n_store into out (outer, out_rows, W).
 // Used for P2, P4 and the y/x passes of the Green's-tensor build.
 // ---------------------------------------------------------------------------------
-struct AxisArgs {
-  int n_outer, outer_per_slot;
-  int in_rows, n_load, out_rows, n_store;
-  long long W;
-  int ncol;
-  int negate;
-};
-
 template <typename T, bool INV, bool GEN>
 __global__ __launch_bounds__(FFT_NT) void k_axis(const C<T>* __restrict__ in, C<T>* __restrict__ out, AxisArgs a,
                                                  FftPlan pl, const C<T>* __restrict__ twl, const SlotCtl* ctl,
@@ -135,8 +132,9 @@ __global__ __launch_bounds__(FFT_NT) void k_axis(const C<T>* __restrict__ in, C<
   load_twiddles<T>(tw, twl, pl.L);
   const C<T>* src = in + (long long)o * a.in_rows * a.W + c0;
   const int nl = a.n_load * ncol;
+  const FastDiv dc = FastDiv::make((unsigned)ncol);
   for (int e = threadIdx.x; e < nl; e += FFT_NT) {
-    const int r = e / ncol, c = e - r * ncol;
+    const int r = (int)dc.div((unsigned)e), c = e - r * ncol;
     data[e] = src[(long long)r * a.W + c];
   }
   const int nz0 = (pl.L - a.n_load) * ncol;
@@ -146,7 +144,7 @@ __global__ __launch_bounds__(FFT_NT) void k_axis(const C<T>* __restrict__ in, C<
   C<T>* dst = out + (long long)o * a.out_rows * a.W + c0;
   const int ns = a.n_store * ncol;
   for (int e = threadIdx.x; e < ns; e += FFT_NT) {
-    const int r = e / ncol, c = e - r * ncol;
+    const int r = (int)dc.div((unsigned)e), c = e - r * ncol;
     C<T> v = data[e];
     if (a.negate) { v.x = -v.x; v.y = -v.y; }
     dst[(long long)r * a.W + c] = v;
@@ -155,7 +153,10 @@ __global__ __launch_bounds__(FFT_NT) void k_axis(const C<T>* __restrict__ in, C<
 
 // ---------------------------------------------------------------------------------
 // P3: x-forward, symmetric 3x3 product with the reduced table, x-inverse, in place.
-// Tile = fixed ky, wz3 consecutive kz (all 3 components), all kx.
+// Tile = the ky mirror pair {q, Ly-q} (one row for q = 0 and q = Ly/2), wz3
+// consecutive kz (all 3 components), all kx. Each reduced-table entry (mx, q, mz) is
+// read once per tile and applied to its four mirror images (kx, Lx-kx) x (ky pair).
+// LDS tile: data[kx * ncol + yrow * cw + 3 * kzl + comp], cw = 3 * wz.
 // ---------------------------------------------------------------------------------
 template <typename T, bool GEN>
 __global__ __launch_bounds__(FFT_NT) void k_xconv(C<T>* __restrict__ B, const C<T>* __restrict__ table,
@@ -165,56 +166,66 @@ __global__ __launch_bounds__(FFT_NT) void k_xconv(C<T>* __restrict__ B, const C<
   extern __shared__ __align__(16) unsigned char smem[];
   C<T>* tw = reinterpret_cast<C<T>*>(smem);
   C<T>* data = tw + g.Lx;
+  const int hx = g.Lx / 2, hy = g.Ly / 2, hz = g.Lz / 2;
   const int nkzb = (g.Lz + g.wz3 - 1) / g.wz3;
   const int t = blockIdx.x;
-  const int slot = t / (g.Ly * nkzb);
-  const int rem = t - slot * (g.Ly * nkzb);
-  const int ky = rem / nkzb, kb = rem - ky * nkzb;
+  const int per_slot = (hy + 1) * nkzb;
+  const int slot = t / per_slot;
+  const int rem = t - slot * per_slot;
+  const int qy = rem / nkzb, kb = rem - qy * nkzb;
   if (!slot_wanted(ctl, slot, need)) return;
-  const int kz0 = kb * g.wz3, wz = min(g.wz3, g.Lz - kz0), ncol = 3 * wz;
+  const int nyr = (qy == 0 || qy == hy) ? 1 : 2;
+  const int kz0 = kb * g.wz3, wz = min(g.wz3, g.Lz - kz0), cw = 3 * wz, ncol = nyr * cw;
   const long long W = 3LL * g.Lz;
   const long long xs = (long long)g.Ly * W;
-  C<T>* base = B + slot * g.b_stride + (long long)ky * W + 3 * kz0;
+  C<T>* base = B + slot * g.b_stride + 3 * kz0;
+  const long long yoff[2] = {(long long)qy * W, (long long)(g.Ly - qy) * W};
   load_twiddles<T>(tw, twx, g.Lx);
   const int nl = g.nx * ncol;
+  const FastDiv dcol = FastDiv::make((unsigned)ncol);
   for (int e = threadIdx.x; e < nl; e += FFT_NT) {
-    const int xr = e / ncol, c = e - xr * ncol;
-    data[e] = base[xr * xs + c];
+    const int xr = (int)dcol.div((unsigned)e), c2 = e - xr * ncol;
+    const int yr = c2 >= cw ? 1 : 0, cc = c2 - yr * cw;
+    data[e] = base[xr * xs + yoff[yr] + cc];
   }
   const int nz0 = (g.Lx - g.nx) * ncol;
   for (int e = threadIdx.x; e < nz0; e += FFT_NT) data[nl + e] = mk<T>(0, 0);
   __syncthreads();
   tile_fft<T, false, GEN>(data, ncol, px, tw);
 
-  const int hx = g.Lx / 2, hy = g.Ly / 2, hz = g.Lz / 2;
-  const bool fy = ky > hy;
-  const int my = fy ? g.Ly - ky : ky;
-  const C<T>* tab = table + slot * table_stride;
-  const int ntrip = g.Lx * wz;
-  for (int e = threadIdx.x; e < ntrip; e += FFT_NT) {
-    const int kx = e / wz, q = e - kx * wz, kz = kz0 + q;
-    const bool fx = kx > hx, fz = kz > hz;
-    const int mx = fx ? g.Lx - kx : kx, mz = fz ? g.Lz - kz : kz;
-    const C<T>* gp = tab + (((long long)mx * (hy + 1) + my) * (hz + 1) + mz) * 6;
-    const C<T> g0 = gp[0], g3 = gp[3], g5 = gp[5];
-    C<T> g1 = gp[1], g2 = gp[2], g4 = gp[4];
-    if (fx != fy) g1 = cscale(g1, T(-1));
-    if (fx != fz) g2 = cscale(g2, T(-1));
-    if (fy != fz) g4 = cscale(g4, T(-1));
-    C<T>* dp = data + kx * ncol + 3 * q;
-    const C<T> x0 = dp[0], x1 = dp[1], x2 = dp[2];
-    const C<T> y0 = cadd(cadd(cmul(g0, x0), cmul(g1, x1)), cmul(g2, x2));
-    const C<T> y1 = cadd(cadd(cmul(g1, x0), cmul(g3, x1)), cmul(g4, x2));
-    const C<T> y2 = cadd(cadd(cmul(g2, x0), cmul(g4, x1)), cmul(g5, x2));
-    dp[0] = cscale(y0, scale);
-    dp[1] = cscale(y1, scale);
-    dp[2] = cscale(y2, scale);
+  const C<T>* tab = table + slot * table_stride + (long long)qy * (hz + 1) * 6;
+  const long long mxs = (long long)(hy + 1) * (hz + 1) * 6;
+  const int nent = (hx + 1) * wz;
+  const FastDiv dwz = FastDiv::make((unsigned)wz);
+  for (int e = threadIdx.x; e < nent; e += FFT_NT) {
+    const int mx = (int)dwz.div((unsigned)e), q = e - mx * wz, kz = kz0 + q;
+    const bool fz = kz > hz;
+    const int mz = fz ? g.Lz - kz : kz;
+    const C<T>* gp = tab + mx * mxs + mz * 6;
+    const C<T> g0 = cscale(gp[0], scale), g1 = cscale(gp[1], scale), g2 = cscale(gp[2], scale);
+    const C<T> g3 = cscale(gp[3], scale), g4 = cscale(gp[4], scale), g5 = cscale(gp[5], scale);
+    const int nkx = (mx == 0 || mx == hx) ? 1 : 2;
+    for (int ix = 0; ix < nkx; ++ix) {
+      const int kx = ix ? g.Lx - mx : mx;
+      const bool fx = ix != 0;
+      for (int yr = 0; yr < nyr; ++yr) {
+        const bool fy = yr != 0;
+        const T sxy = (fx != fy) ? T(-1) : T(1), sxz = (fx != fz) ? T(-1) : T(1), syz = (fy != fz) ? T(-1) : T(1);
+        const C<T> h1 = cscale(g1, sxy), h2 = cscale(g2, sxz), h4 = cscale(g4, syz);
+        C<T>* dp = data + kx * ncol + yr * cw + 3 * q;
+        const C<T> x0 = dp[0], x1 = dp[1], x2 = dp[2];
+        dp[0] = cadd(cadd(cmul(g0, x0), cmul(h1, x1)), cmul(h2, x2));
+        dp[1] = cadd(cadd(cmul(h1, x0), cmul(g3, x1)), cmul(h4, x2));
+        dp[2] = cadd(cadd(cmul(h2, x0), cmul(h4, x1)), cmul(g5, x2));
+      }
+    }
   }
   __syncthreads();
   tile_fft<T, true, GEN>(data, ncol, px, tw);
   for (int e = threadIdx.x; e < nl; e += FFT_NT) {
-    const int xr = e / ncol, c = e - xr * ncol;
-    base[xr * xs + c] = data[e];
+    const int xr = (int)dcol.div((unsigned)e), c2 = e - xr * ncol;
+    const int yr = c2 >= cw ? 1 : 0, cc = c2 - yr * cw;
+    base[xr * xs + yoff[yr] + cc] = data[e];
   }
 }
 
@@ -242,8 +253,9 @@ __global__ __launch_bounds__(FFT_NT) void k_zinv(const C<T>* __restrict__ A, C<T
   load_twiddles<T>(tw, twz, g.Lz);
   const C<T>* src = A + slot * g.a_stride + ((long long)ix * g.ny + y0) * g.Lz * 3;
   const int ri = 3 * g.Lz, nin = rows * ri;
+  const FastDiv dri = FastDiv::make((unsigned)ri);
   for (int e = threadIdx.x; e < nin; e += FFT_NT) {
-    const int row = e / ri, r2 = e - row * ri, p = r2 / 3, c = r2 - 3 * p;
+    const int row = (int)dri.div((unsigned)e), r2 = e - row * ri, p = r2 / 3, c = r2 - 3 * p;
     data[p * ncol + row * 3 + c] = src[e];
   }
   __syncthreads();
@@ -258,8 +270,9 @@ __global__ __launch_bounds__(FFT_NT) void k_zinv(const C<T>* __restrict__ A, C<T
   const long long vbase = slot * g.vec_stride + site0 * 3;
   const long long sbase = slot * site_stride + site0;
   const int ro = 3 * g.nz, nout = rows * ro;
+  const FastDiv dro = FastDiv::make((unsigned)ro);
   for (int e = threadIdx.x; e < nout; e += FFT_NT) {
-    const int row = e / ro, r2 = e - row * ro, p = r2 / 3, c = r2 - 3 * p;
+    const int row = (int)dro.div((unsigned)e), r2 = e - row * ro, p = r2 / 3, c = r2 - 3 * p;
     const C<T> cv = data[p * ncol + row * 3 + c];
     const long long sidx = sbase + (long long)row * g.nz + p;
     const long long vidx = vbase + e;
@@ -318,8 +331,9 @@ __global__ __launch_bounds__(FFT_NT) void k_green_z(double2* __restrict__ G1, in
   load_twiddles<double>(tw, twz, Lz);
   const double px = (ix >= nx) ? (ix - 2 * nx) * d : ix * d;
   const int n = rows * Lz;
+  const FastDiv dlz = FastDiv::make((unsigned)Lz);
   for (int e = threadIdx.x; e < n; e += FFT_NT) {
-    const int row = e / Lz, iz = e - row * Lz, iy = y0 + row;
+    const int row = (int)dlz.div((unsigned)e), iz = e - row * Lz, iy = y0 + row;
     const double py = (iy >= ny) ? (iy - 2 * ny) * d : iy * d;
     const double pzv = (iz >= nz) ? (iz - 2 * nz) * d : iz * d;
     double2 val[6];
@@ -425,8 +439,11 @@ int make_fft_plan(int L, FftPlan* out) {
   out->L = L;
   out->npass = (int)rs.size();
   out->gen = 0;
+  int ns = 1;
   for (int i = 0; i < out->npass; ++i) {
     out->radix[i] = rs[i];
+    out->ns_div[i] = FastDiv::make((unsigned)ns);
+    ns *= rs[i];
     if (rs[i] == 7 || rs[i] == 11 || rs[i] == 13) out->gen = 1;
   }
   return PDDA_OK;
@@ -470,7 +487,7 @@ int conv_plan_init(ConvPlanBase* p, int prec, int nx, int ny, int nz, int hx, in
   if ((rc = make_fft_plan(g.Lz, &p->pz))) return rc;
   const int cap = prec ? tile_cap<double>() : tile_cap<float>();
   const size_t es = prec ? sizeof(double2) : sizeof(float2);
-  if (3 * g.Lz > cap || 3 * g.Lx > cap || g.Ly > cap) {
+  if (3 * g.Lz > cap || 6 * g.Lx > cap || g.Ly > cap) {
     set_error("FFT length too long for one LDS tile at this precision");
     return PDDA_ERR_SIZE;
   }
@@ -488,10 +505,10 @@ int conv_plan_init(ConvPlanBase* p, int prec, int nx, int ny, int nz, int hx, in
     long long want = std::max(16LL, W * nx * nslots / target);
     g.ncol2 = (int)std::min<long long>(cmax, want);
   }
-  // P3: kz per tile
+  // P3: kz per tile (tiles hold the ky mirror pair, so two rows of 3*wz3 columns)
   {
-    int wmax = std::min(g.Lz, cap / (3 * g.Lx));
-    long long want = std::max(1LL, (long long)g.Lz * g.Ly * nslots / target);
+    int wmax = std::min(g.Lz, cap / (6 * g.Lx));
+    long long want = std::max(1LL, (long long)g.Lz * (g.Ly / 2 + 1) * nslots / target);
     g.wz3 = (int)std::max(1LL, std::min((long long)wmax, want));
   }
   g.vec_stride = 3LL * nx * ny * nz;
@@ -504,12 +521,13 @@ int conv_plan_init(ConvPlanBase* p, int prec, int nx, int ny, int nz, int hx, in
   PDDA_HIP_CHECK(hipMalloc(&p->work_b, (size_t)g.b_stride * nslots * es));
   const int tiles_y = ceil_div(ny, g.rows1);
   p->nwg5 = nx * tiles_y;
-  PDDA_HIP_CHECK(hipMalloc(&p->partial, (size_t)nslots * p->nwg5 * 6 * sizeof(cd)));
+  fast_setup(p);  // may switch P5 to a fast tile (and so change nwg5)
+  PDDA_HIP_CHECK(hipMalloc(&p->partial, (size_t)nslots * std::max(p->nwg5, nx * tiles_y) * 6 * sizeof(cd)));
   PDDA_HIP_CHECK(hipMalloc(&p->counters, (size_t)nslots * sizeof(int)));
   PDDA_HIP_CHECK(hipMemset(p->counters, 0, (size_t)nslots * sizeof(int)));
   p->lds[0] = (size_t)(g.Lz + g.Lz * 3 * g.rows1) * es;
   p->lds[1] = (size_t)(g.Ly + g.Ly * g.ncol2) * es;
-  p->lds[2] = (size_t)(g.Lx + g.Lx * 3 * g.wz3) * es;
+  p->lds[2] = (size_t)(g.Lx + g.Lx * 6 * g.wz3) * es;
   p->lds[3] = p->lds[1];
   p->lds[4] = p->lds[0];
   return PDDA_OK;
@@ -545,24 +563,30 @@ static void launch_p5(ConvPlanBase* p, const ConvArgs& a, hipStream_t s) {
 
 template <typename T, int MODE>
 static void launch_p5_dots(ConvPlanBase* p, const ConvArgs& a, hipStream_t s) {
+  switch (a.dots) {
+    case DOT_ALPHA: launch_p5<T, MODE, DOT_ALPHA>(p, a, s); break;
+    case DOT_ZETA: launch_p5<T, MODE, DOT_ZETA>(p, a, s); break;
+    case DOT_RESID: launch_p5<T, MODE, DOT_RESID>(p, a, s); break;
+    default: launch_p5<T, MODE, DOT_NONE>(p, a, s); break;
+  }
+}
+
+static void launch_finalize(ConvPlanBase* p, const ConvArgs& a, hipStream_t s) {
   const int nsl = p->g.nslots, stride = p->nwg5 * 6;
   switch (a.dots) {
     case DOT_ALPHA:
-      launch_p5<T, MODE, DOT_ALPHA>(p, a, s);
       hipLaunchKernelGGL(k_finalize_dots<1>, dim3(nsl), dim3(FFT_NT), 0, s, (const cd*)p->partial, p->nwg5, stride,
-                         a.ctl, (int)DOT_ALPHA, a.need);
+                         a.ctl, (int)DOT_ALPHA, a.need, 0);
       break;
     case DOT_ZETA:
-      launch_p5<T, MODE, DOT_ZETA>(p, a, s);
       hipLaunchKernelGGL(k_finalize_dots<6>, dim3(nsl), dim3(FFT_NT), 0, s, (const cd*)p->partial, p->nwg5, stride,
-                         a.ctl, (int)DOT_ZETA, a.need);
+                         a.ctl, (int)DOT_ZETA, a.need, 0);
       break;
     case DOT_RESID:
-      launch_p5<T, MODE, DOT_RESID>(p, a, s);
       hipLaunchKernelGGL(k_finalize_dots<2>, dim3(nsl), dim3(FFT_NT), 0, s, (const cd*)p->partial, p->nwg5, stride,
-                         a.ctl, (int)DOT_RESID, a.need);
+                         a.ctl, (int)DOT_RESID, a.need, a.fin_control ? 1 : 0);
       break;
-    default: launch_p5<T, MODE, DOT_NONE>(p, a, s); break;
+    default: break;
   }
 }
 
@@ -573,7 +597,9 @@ static int conv_apply_t(ConvPlanBase* p, const ConvArgs& a, hipStream_t s, hipEv
   C<T>* A = (C<T>*)p->work_a;
   C<T>* B = (C<T>*)p->work_b;
   // P1
-  {
+  if (p->fast[0]) {
+    fast_launch(p, 0, a, s);
+  } else {
     const int nblk = g.nslots * g.nx * ceil_div(g.ny, g.rows1);
     if (p->pz.gen)
       hipLaunchKernelGGL((k_zfwd<T, true>), dim3(nblk), dim3(FFT_NT), p->lds[0], s, (const C<T>*)a.x, A, g, p->pz,
@@ -584,7 +610,9 @@ static int conv_apply_t(ConvPlanBase* p, const ConvArgs& a, hipStream_t s, hipEv
   }
   if (ev) PDDA_HIP_CHECK(hipEventRecord(ev[1], s));
   // P2
-  {
+  if (p->fast[1]) {
+    fast_launch(p, 1, a, s);
+  } else {
     AxisArgs ax;
     ax.n_outer = g.nslots * g.nx; ax.outer_per_slot = g.nx;
     ax.in_rows = g.ny; ax.n_load = g.ny; ax.out_rows = g.Ly; ax.n_store = g.Ly;
@@ -599,8 +627,10 @@ static int conv_apply_t(ConvPlanBase* p, const ConvArgs& a, hipStream_t s, hipEv
   }
   if (ev) PDDA_HIP_CHECK(hipEventRecord(ev[2], s));
   // P3
-  {
-    const int nblk = g.nslots * g.Ly * ceil_div(g.Lz, g.wz3);
+  if (p->fast[2]) {
+    fast_launch(p, 2, a, s);
+  } else {
+    const int nblk = g.nslots * (g.Ly / 2 + 1) * ceil_div(g.Lz, g.wz3);
     const T scale = T(1.0 / ((double)g.Lx * g.Ly * g.Lz));
     if (p->px.gen)
       hipLaunchKernelGGL((k_xconv<T, true>), dim3(nblk), dim3(FFT_NT), p->lds[2], s, B, (const C<T>*)a.table,
@@ -611,7 +641,9 @@ static int conv_apply_t(ConvPlanBase* p, const ConvArgs& a, hipStream_t s, hipEv
   }
   if (ev) PDDA_HIP_CHECK(hipEventRecord(ev[3], s));
   // P4
-  {
+  if (p->fast[3]) {
+    fast_launch(p, 3, a, s);
+  } else {
     AxisArgs ax;
     ax.n_outer = g.nslots * g.nx; ax.outer_per_slot = g.nx;
     ax.in_rows = g.Ly; ax.n_load = g.Ly; ax.out_rows = g.ny; ax.n_store = g.ny;
@@ -626,13 +658,18 @@ static int conv_apply_t(ConvPlanBase* p, const ConvArgs& a, hipStream_t s, hipEv
   }
   if (ev) PDDA_HIP_CHECK(hipEventRecord(ev[4], s));
   // P5
-  switch (a.mode) {
-    case MODE_FIELD: launch_p5_dots<T, MODE_FIELD>(p, a, s); break;
-    case MODE_PRECOND: launch_p5_dots<T, MODE_PRECOND>(p, a, s); break;
-    case MODE_INTERACT: launch_p5_dots<T, MODE_INTERACT>(p, a, s); break;
-    case MODE_RESIDUAL: launch_p5_dots<T, MODE_RESIDUAL>(p, a, s); break;
-    default: set_error("unknown convolution mode"); return PDDA_ERR_ARG;
+  if (a.mode < MODE_FIELD || a.mode > MODE_RESIDUAL) { set_error("unknown convolution mode"); return PDDA_ERR_ARG; }
+  if (p->fast[4]) {
+    fast_launch(p, 4, a, s);
+  } else {
+    switch (a.mode) {
+      case MODE_FIELD: launch_p5_dots<T, MODE_FIELD>(p, a, s); break;
+      case MODE_PRECOND: launch_p5_dots<T, MODE_PRECOND>(p, a, s); break;
+      case MODE_INTERACT: launch_p5_dots<T, MODE_INTERACT>(p, a, s); break;
+      default: launch_p5_dots<T, MODE_RESIDUAL>(p, a, s); break;
+    }
   }
+  launch_finalize(p, a, s);
   if (ev) PDDA_HIP_CHECK(hipEventRecord(ev[5], s));
   PDDA_LAUNCH_CHECK();
   return PDDA_OK;

@@ -1,0 +1,420 @@
+// Convolution passes on the compile-time-planned FFT (lengths 16 ... 512).
+//   P2/P4 (y) and P3 (x): the first FFT pass loads its rows straight from HBM and the
+//   last pass stores straight to HBM (pruned to the nonzero / kept rows).
+//   P1/P5 (z): rows are contiguous in z, so the tile is staged through LDS on both
+//   sides to keep the HBM accesses contiguous.
+#include <algorithm>
+#include <cstdlib>
+
+#include "conv.h"
+#include "fft_ct.h"
+
+namespace pdda {
+
+namespace {
+
+template <typename T> __device__ inline C<T> zero() { return mk<T>(T(0), T(0)); }
+
+// ------------------------------------------------------------------ P1: z-forward
+template <typename T, int L>
+__global__ __launch_bounds__(FFT_NT) void kf_zfwd(const C<T>* __restrict__ x, C<T>* __restrict__ A, ConvGeom g,
+                                                  const C<T>* __restrict__ twz, const SlotCtl* ctl, int need,
+                                                  int rows1) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  C<T>* tw = reinterpret_cast<C<T>*>(smem);
+  C<T>* lds = tw + L;
+  const int tiles_y = (g.ny + rows1 - 1) / rows1;
+  const int t = blockIdx.x;
+  const int slot = t / (g.nx * tiles_y);
+  const int rem = t - slot * (g.nx * tiles_y);
+  const int ix = rem / tiles_y, ty = rem - ix * tiles_y;
+  if (!slot_wanted(ctl, slot, need)) return;
+  const int y0 = ty * rows1, rows = min(rows1, g.ny - y0);
+  const int C3 = 3 * rows1;
+  load_twiddles<T>(tw, twz, L);
+  const C<T>* src = x + slot * g.vec_stride + ((long long)ix * g.ny + y0) * g.nz * 3;
+  const int rl = 3 * g.nz, nin = rows * rl;
+  const FastDiv drl = FastDiv::make((unsigned)rl);
+  for (int e = threadIdx.x; e < nin; e += FFT_NT) {
+    const int row = (int)drl.div((unsigned)e), r2 = e - row * rl, p = r2 / 3, c = r2 - 3 * p;
+    lds[p * C3 + row * 3 + c] = src[e];
+  }
+  __syncthreads();
+  const ColMap cm = ColMap::make(C3);
+  const int nz = g.nz, ncv = 3 * rows;
+  auto ld0 = [&](int pos, int col) { return (pos < nz && col < ncv) ? lds[pos * C3 + col] : zero<T>(); };
+  auto stN = [&](int pos, int col, C<T> v) { lds[pos * C3 + col] = v; };
+  ct_fft<T, L, false, true>(cm, tw, lds, ld0, stN);
+  __syncthreads();
+  C<T>* dst = A + slot * g.a_stride + ((long long)ix * g.ny + y0) * L * 3;
+  constexpr int ro = 3 * L;
+  const int nout = rows * ro;
+  for (int e = threadIdx.x; e < nout; e += FFT_NT) {
+    const int row = e / ro, r2 = e - row * ro, p = r2 / 3, c = r2 - 3 * p;
+    dst[e] = lds[p * C3 + row * 3 + c];
+  }
+}
+
+// ------------------------------------------------------------------ P2/P4: y axis
+template <typename T, int L, bool INV>
+__global__ __launch_bounds__(FFT_NT) void kf_axis(const C<T>* __restrict__ in, C<T>* __restrict__ out, AxisArgs a,
+                                                  const C<T>* __restrict__ twl, const SlotCtl* ctl, int need) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  C<T>* tw = reinterpret_cast<C<T>*>(smem);
+  C<T>* lds = tw + L;
+  const long long ncb = (a.W + a.ncol - 1) / a.ncol;
+  const long long t = blockIdx.x;
+  const int o = (int)(t / ncb);
+  const long long c0 = (t - (long long)o * ncb) * a.ncol;
+  if (!slot_wanted(ctl, o / a.outer_per_slot, need)) return;
+  const int nce = (int)min((long long)a.ncol, a.W - c0);
+  load_twiddles<T>(tw, twl, L);
+  const ColMap cm = ColMap::make(a.ncol);
+  const C<T>* src = in + (long long)o * a.in_rows * a.W + c0;
+  C<T>* dst = out + (long long)o * a.out_rows * a.W + c0;
+  const long long W = a.W;
+  const int nl = a.n_load, ns = a.n_store;
+  auto ld0 = [&](int pos, int col) { return (pos < nl && col < nce) ? src[pos * W + col] : zero<T>(); };
+  auto stN = [&](int pos, int col, C<T> v) {
+    if (pos < ns && col < nce) dst[pos * W + col] = v;
+  };
+  ct_fft<T, L, INV, false>(cm, tw, lds, ld0, stN);
+}
+
+// ------------------------------------------------------------------ P3: x conv
+template <typename T, int L>
+__global__ __launch_bounds__(FFT_NT) void kf_xconv(C<T>* __restrict__ B, const C<T>* __restrict__ table,
+                                                   long long table_stride, ConvGeom g,
+                                                   const C<T>* __restrict__ twx, const SlotCtl* ctl, int need,
+                                                   T scale, int wz3) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  C<T>* tw = reinterpret_cast<C<T>*>(smem);
+  C<T>* lds = tw + L;
+  constexpr int hx = L / 2;
+  const int hy = g.Ly / 2, hz = g.Lz / 2;
+  const int nkzb = (g.Lz + wz3 - 1) / wz3;
+  const int t = blockIdx.x;
+  const int per_slot = (hy + 1) * nkzb;
+  const int slot = t / per_slot;
+  const int rem = t - slot * per_slot;
+  const int qy = rem / nkzb, kb = rem - qy * nkzb;
+  if (!slot_wanted(ctl, slot, need)) return;
+  const int nyr = (qy == 0 || qy == hy) ? 1 : 2;
+  const int kz0 = kb * wz3, wz = min(wz3, g.Lz - kz0);
+  const int cw = 3 * wz3, nc = nyr * cw, cwe = 3 * wz;
+  const long long W = 3LL * g.Lz;
+  const long long xs = (long long)g.Ly * W;
+  C<T>* base = B + slot * g.b_stride + 3 * kz0;
+  const long long y0 = (long long)qy * W, y1 = (long long)(g.Ly - qy) * W;
+  load_twiddles<T>(tw, twx, L);
+  const ColMap cm = ColMap::make(nc);
+  const int nx = g.nx;
+  auto gaddr = [&](int pos, int col, bool& ok) -> long long {
+    const int yr = col >= cw ? 1 : 0, cc = col - yr * cw;
+    ok = cc < cwe;
+    return pos * xs + (yr ? y1 : y0) + cc;
+  };
+  auto ld0 = [&](int pos, int col) {
+    bool ok;
+    const long long ad = gaddr(pos, col, ok);
+    return (pos < nx && ok) ? base[ad] : zero<T>();
+  };
+  auto stl = [&](int pos, int col, C<T> v) { lds[pos * nc + col] = v; };
+  ct_fft<T, L, false, false>(cm, tw, lds, ld0, stl);
+  __syncthreads();
+
+  const C<T>* tab = table + slot * table_stride + (long long)qy * (hz + 1) * 6;
+  const long long mxs = (long long)(hy + 1) * (hz + 1) * 6;
+  const int nent = (hx + 1) * wz;
+  const FastDiv dwz = FastDiv::make((unsigned)wz);
+  for (int e = threadIdx.x; e < nent; e += FFT_NT) {
+    const int mx = (int)dwz.div((unsigned)e), q = e - mx * wz, kz = kz0 + q;
+    const bool fz = kz > hz;
+    const int mz = fz ? g.Lz - kz : kz;
+    const C<T>* gp = tab + mx * mxs + mz * 6;
+    const C<T> g0 = cscale(gp[0], scale), g1 = cscale(gp[1], scale), g2 = cscale(gp[2], scale);
+    const C<T> g3 = cscale(gp[3], scale), g4 = cscale(gp[4], scale), g5 = cscale(gp[5], scale);
+    const int nkx = (mx == 0 || mx == hx) ? 1 : 2;
+    for (int ix = 0; ix < nkx; ++ix) {
+      const int kx = ix ? L - mx : mx;
+      const bool fx = ix != 0;
+      for (int yr = 0; yr < nyr; ++yr) {
+        const bool fy = yr != 0;
+        const T sxy = (fx != fy) ? T(-1) : T(1), sxz = (fx != fz) ? T(-1) : T(1), syz = (fy != fz) ? T(-1) : T(1);
+        const C<T> h1 = cscale(g1, sxy), h2 = cscale(g2, sxz), h4 = cscale(g4, syz);
+        C<T>* dp = lds + kx * nc + yr * cw + 3 * q;
+        const C<T> x0 = dp[0], x1 = dp[1], x2 = dp[2];
+        dp[0] = cadd(cadd(cmul(g0, x0), cmul(h1, x1)), cmul(h2, x2));
+        dp[1] = cadd(cadd(cmul(h1, x0), cmul(g3, x1)), cmul(h4, x2));
+        dp[2] = cadd(cadd(cmul(h2, x0), cmul(h4, x1)), cmul(g5, x2));
+      }
+    }
+  }
+  __syncthreads();
+  auto ldl = [&](int pos, int col) { return lds[pos * nc + col]; };
+  auto stN = [&](int pos, int col, C<T> v) {
+    bool ok;
+    const long long ad = gaddr(pos, col, ok);
+    if (pos < nx && ok) base[ad] = v;
+  };
+  ct_fft<T, L, true, true>(cm, tw, lds, ldl, stN);
+}
+
+// ------------------------------------------------------------------ P5: z-inverse + epilogue
+template <typename T, int L, int DOTS>
+__global__ __launch_bounds__(FFT_NT) void kf_zinv(const C<T>* __restrict__ A, C<T>* __restrict__ out, ConvGeom g,
+                                                  const C<T>* __restrict__ twz, int mode,
+                                                  const C<T>* __restrict__ xin, const C<T>* __restrict__ ia,
+                                                  const uint8_t* __restrict__ mask, long long site_stride,
+                                                  const C<T>* __restrict__ bvec, const C<T>* __restrict__ d0,
+                                                  const C<T>* __restrict__ d1, cd* partial, const SlotCtl* ctl,
+                                                  int need, int nwg, int rows1) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  C<T>* tw = reinterpret_cast<C<T>*>(smem);
+  C<T>* lds = tw + L;
+  const int tiles_y = (g.ny + rows1 - 1) / rows1;
+  const int t = blockIdx.x;
+  const int slot = t / (g.nx * tiles_y);
+  const int wg = t - slot * (g.nx * tiles_y);
+  const int ix = wg / tiles_y, ty = wg - ix * tiles_y;
+  if (!slot_wanted(ctl, slot, need)) return;
+  const int y0 = ty * rows1, rows = min(rows1, g.ny - y0);
+  const int C3 = 3 * rows1;
+  load_twiddles<T>(tw, twz, L);
+  const C<T>* src = A + slot * g.a_stride + ((long long)ix * g.ny + y0) * L * 3;
+  constexpr int ri = 3 * L;
+  const int nin = rows * ri;
+  for (int e = threadIdx.x; e < nin; e += FFT_NT) {
+    const int row = e / ri, r2 = e - row * ri, p = r2 / 3, c = r2 - 3 * p;
+    lds[p * C3 + row * 3 + c] = src[e];
+  }
+  __syncthreads();
+  const ColMap cm = ColMap::make(C3);
+  const int nz = g.nz;
+  auto ldl = [&](int pos, int col) { return lds[pos * C3 + col]; };
+  auto stN = [&](int pos, int col, C<T> v) {
+    if (pos < nz) lds[pos * C3 + col] = v;
+  };
+  ct_fft<T, L, true, true>(cm, tw, lds, ldl, stN);
+  __syncthreads();
+
+  constexpr int ND = DOTS == DOT_ALPHA ? 1 : DOTS == DOT_ZETA ? 6 : DOTS == DOT_RESID ? 2 : 1;
+  double sre[ND], sim[ND];
+#pragma unroll
+  for (int d = 0; d < ND; ++d) { sre[d] = 0.0; sim[d] = 0.0; }
+  const long long site0 = ((long long)ix * g.ny + y0) * g.nz;
+  const long long vbase = slot * g.vec_stride + site0 * 3;
+  const long long sbase = slot * site_stride + site0;
+  const int ro = 3 * g.nz, nout = rows * ro;
+  const FastDiv dro = FastDiv::make((unsigned)ro);
+  for (int e = threadIdx.x; e < nout; e += FFT_NT) {
+    const int row = (int)dro.div((unsigned)e), r2 = e - row * ro, p = r2 / 3, c = r2 - 3 * p;
+    const C<T> cv = lds[p * C3 + row * 3 + c];
+    const long long sidx = sbase + (long long)row * g.nz + p;
+    const long long vidx = vbase + e;
+    C<T> o;
+    if (mode == MODE_FIELD) {
+      o = cv;
+    } else {
+      const C<T> mc = cscale(cv, mask[sidx] ? T(1) : T(0));
+      if (mode == MODE_PRECOND) {
+        o = mc;
+      } else {
+        const C<T> v = csub(cmul(ia[sidx], xin[vidx]), mc);
+        o = (mode == MODE_INTERACT) ? v : csub(bvec[vidx], v);
+      }
+    }
+    out[vidx] = o;
+    if (DOTS == DOT_ALPHA) {
+      const C<T> a = d0[vidx];
+      sre[0] += (double)a.x * o.x + (double)a.y * o.y;
+      sim[0] += (double)a.x * o.y - (double)a.y * o.x;
+    } else if (DOTS == DOT_RESID) {
+      const C<T> a = d0[vidx];
+      sre[0] += (double)o.x * o.x + (double)o.y * o.y;
+      sre[1] += (double)a.x * o.x + (double)a.y * o.y;
+      sim[1] += (double)a.x * o.y - (double)a.y * o.x;
+    } else if (DOTS == DOT_ZETA) {
+      const C<T> y = d0[vidx], s = d1[vidx];
+      const double yx = y.x, yy = y.y, tx = o.x, ty2 = o.y, sx = s.x, sy = s.y;
+      sre[0] += yx * yx + yy * yy;
+      sre[1] += tx * tx + ty2 * ty2;
+      sre[2] += yx * tx + yy * ty2; sim[2] += yx * ty2 - yy * tx;
+      sre[3] += tx * yx + ty2 * yy; sim[3] += tx * yy - ty2 * yx;
+      sre[4] += yx * sx + yy * sy; sim[4] += yx * sy - yy * sx;
+      sre[5] += tx * sx + ty2 * sy; sim[5] += tx * sy - ty2 * sx;
+    }
+  }
+  if (DOTS != DOT_NONE) {
+    __shared__ double scratch[2 * ND * (FFT_NT / 64)];
+    block_sum<ND>(sre, sim, scratch);
+    if (threadIdx.x == 0) {
+      cd* pp = partial + (size_t)slot * nwg * 6 + (size_t)wg * ND;
+      for (int d = 0; d < ND; ++d) pp[d] = cd{sre[d], sim[d]};
+    }
+  }
+}
+
+// ------------------------------------------------------------------ host dispatch
+bool has_plan(int L) {
+  switch (L) {
+    case 16: case 24: case 32: case 48: case 64: case 96: case 128: case 192: case 256: case 384: case 512:
+      return true;
+    default:
+      return false;
+  }
+}
+
+int ept_of(int prec, int L) { return prec ? (L >= 384 ? 16 : 12) : (L >= 384 ? 32 : 24); }
+
+// Largest power-of-two multiplier p <= want with C = c3 * p meeting the register bound
+// (256 / C >= ceil(L / EPT)) and the LDS budget; 0 if none.
+int pick_pow2(int L, int prec, int c3, int want, int pmax) {
+  const int ept = ept_of(prec, L);
+  const int need_js = (L + ept - 1) / ept;
+  const size_t es = prec ? 16 : 8;
+  int best = 0;
+  for (int p = 1; p <= pmax && p <= want; p *= 2) {
+    const int C = c3 * p;
+    if (C > FFT_NT || FFT_NT / C < need_js) break;
+    if ((size_t)(L + (size_t)L * C) * es > 60 * 1024) break;
+    best = p;
+  }
+  return best;
+}
+
+template <typename T, int L> void launch_p1(ConvPlanBase* p, const ConvArgs& a, hipStream_t s) {
+  const ConvGeom& g = p->g;
+  const int nblk = g.nslots * g.nx * ((g.ny + p->f_rows - 1) / p->f_rows);
+  hipLaunchKernelGGL((kf_zfwd<T, L>), dim3(nblk), dim3(FFT_NT), p->f_lds[0], s, (const C<T>*)a.x,
+                     (C<T>*)p->work_a, g, (const C<T>*)p->tw[2], a.ctl, a.need, p->f_rows);
+}
+
+template <typename T, int L, bool INV> void launch_axis(ConvPlanBase* p, const ConvArgs& a, hipStream_t s) {
+  const ConvGeom& g = p->g;
+  AxisArgs ax;
+  ax.n_outer = g.nslots * g.nx;
+  ax.outer_per_slot = g.nx;
+  ax.W = 3LL * g.Lz;
+  ax.ncol = p->f_ncol;
+  ax.negate = 0;
+  if (!INV) { ax.in_rows = g.ny; ax.n_load = g.ny; ax.out_rows = L; ax.n_store = L; }
+  else { ax.in_rows = L; ax.n_load = L; ax.out_rows = g.ny; ax.n_store = g.ny; }
+  const long long nblk = (long long)ax.n_outer * ((ax.W + ax.ncol - 1) / ax.ncol);
+  const C<T>* in = (const C<T>*)(INV ? p->work_b : p->work_a);
+  C<T>* out = (C<T>*)(INV ? p->work_a : p->work_b);
+  hipLaunchKernelGGL((kf_axis<T, L, INV>), dim3((unsigned)nblk), dim3(FFT_NT), p->f_lds[INV ? 3 : 1], s, in, out,
+                     ax, (const C<T>*)p->tw[1], a.ctl, a.need);
+}
+
+template <typename T, int L> void launch_p3(ConvPlanBase* p, const ConvArgs& a, hipStream_t s) {
+  const ConvGeom& g = p->g;
+  const int nblk = g.nslots * (g.Ly / 2 + 1) * ((g.Lz + p->f_wz - 1) / p->f_wz);
+  const T scale = T(1.0 / ((double)g.Lx * g.Ly * g.Lz));
+  hipLaunchKernelGGL((kf_xconv<T, L>), dim3(nblk), dim3(FFT_NT), p->f_lds[2], s, (C<T>*)p->work_b,
+                     (const C<T>*)a.table, a.table_stride, g, (const C<T>*)p->tw[0], a.ctl, a.need, scale, p->f_wz);
+}
+
+template <typename T, int L, int DOTS> void launch_p5d(ConvPlanBase* p, const ConvArgs& a, hipStream_t s) {
+  const ConvGeom& g = p->g;
+  const int nblk = g.nslots * p->nwg5;
+  hipLaunchKernelGGL((kf_zinv<T, L, DOTS>), dim3(nblk), dim3(FFT_NT), p->f_lds[4], s, (const C<T>*)p->work_a,
+                     (C<T>*)a.out, g, (const C<T>*)p->tw[2], a.mode, (const C<T>*)a.x, (const C<T>*)a.inv_alpha,
+                     a.mask, a.site_stride, (const C<T>*)a.b, (const C<T>*)a.d0, (const C<T>*)a.d1, p->partial,
+                     a.ctl, a.need, p->nwg5, p->f_rows);
+}
+
+template <typename T, int L> void launch_p5(ConvPlanBase* p, const ConvArgs& a, hipStream_t s) {
+  switch (a.dots) {
+    case DOT_ALPHA: launch_p5d<T, L, DOT_ALPHA>(p, a, s); break;
+    case DOT_ZETA: launch_p5d<T, L, DOT_ZETA>(p, a, s); break;
+    case DOT_RESID: launch_p5d<T, L, DOT_RESID>(p, a, s); break;
+    default: launch_p5d<T, L, DOT_NONE>(p, a, s); break;
+  }
+}
+
+template <typename T, int L> void launch_pass(ConvPlanBase* p, int pass, const ConvArgs& a, hipStream_t s) {
+  switch (pass) {
+    case 0: launch_p1<T, L>(p, a, s); break;
+    case 1: launch_axis<T, L, false>(p, a, s); break;
+    case 2: launch_p3<T, L>(p, a, s); break;
+    case 3: launch_axis<T, L, true>(p, a, s); break;
+    default: launch_p5<T, L>(p, a, s); break;
+  }
+}
+
+template <typename T> void launch_pass_L(ConvPlanBase* p, int pass, int L, const ConvArgs& a, hipStream_t s) {
+  switch (L) {
+    case 16: launch_pass<T, 16>(p, pass, a, s); break;
+    case 24: launch_pass<T, 24>(p, pass, a, s); break;
+    case 32: launch_pass<T, 32>(p, pass, a, s); break;
+    case 48: launch_pass<T, 48>(p, pass, a, s); break;
+    case 64: launch_pass<T, 64>(p, pass, a, s); break;
+    case 96: launch_pass<T, 96>(p, pass, a, s); break;
+    case 128: launch_pass<T, 128>(p, pass, a, s); break;
+    case 192: launch_pass<T, 192>(p, pass, a, s); break;
+    case 256: launch_pass<T, 256>(p, pass, a, s); break;
+    case 384: launch_pass<T, 384>(p, pass, a, s); break;
+    case 512: launch_pass<T, 512>(p, pass, a, s); break;
+    default: break;
+  }
+}
+
+}  // namespace
+
+void fast_setup(ConvPlanBase* p) {
+  ConvGeom& g = p->g;
+  const int prec = p->prec;
+  const size_t es = prec ? 16 : 8;
+  const long long target = 2048;
+  for (int i = 0; i < 5; ++i) p->fast[i] = 0;
+  if (const char* e = getenv("PDDA_DISABLE_FAST"); e && e[0] == '1') return;
+  // P1/P5: C = 3 * rows
+  if (has_plan(g.Lz)) {
+    long long want = std::max(1LL, (long long)g.nx * g.ny * g.nslots / target);
+    int pmax = 1;
+    while (pmax * 2 <= g.ny) pmax *= 2;
+    const int r = pick_pow2(g.Lz, prec, 3, (int)std::min<long long>(want, 1 << 20), pmax);
+    if (r > 0) {
+      p->f_rows = r;
+      p->f_lds[0] = p->f_lds[4] = (size_t)(g.Lz + (size_t)g.Lz * 3 * r) * es;
+      p->fast[0] = p->fast[4] = 1;
+    }
+  }
+  // P2/P4: C power of two
+  if (has_plan(g.Ly)) {
+    const long long W = 3LL * g.Lz;
+    long long want = std::max(16LL, W * g.nx * g.nslots / target);
+    int pmax = 1;
+    while (pmax * 2 <= W) pmax *= 2;
+    const int c = pick_pow2(g.Ly, prec, 1, (int)std::min<long long>(want, 1 << 20), std::max(pmax, 1));
+    if (c > 0) {
+      p->f_ncol = c;
+      p->f_lds[1] = p->f_lds[3] = (size_t)(g.Ly + (size_t)g.Ly * c) * es;
+      p->fast[1] = p->fast[3] = 1;
+    }
+  }
+  // P3: C = 6 * wz (two ky rows)
+  if (has_plan(g.Lx)) {
+    long long want = std::max(1LL, (long long)g.Lz * (g.Ly / 2 + 1) * g.nslots / target);
+    int pmax = 1;
+    while (pmax * 2 <= g.Lz) pmax *= 2;
+    const int w = pick_pow2(g.Lx, prec, 6, (int)std::min<long long>(want, 1 << 20), pmax);
+    if (w > 0) {
+      p->f_wz = w;
+      p->f_lds[2] = (size_t)(g.Lx + (size_t)g.Lx * 6 * w) * es;
+      p->fast[2] = 1;
+    }
+  }
+  if (p->fast[0]) p->nwg5 = g.nx * ((g.ny + p->f_rows - 1) / p->f_rows);
+}
+
+int fast_launch(ConvPlanBase* p, int pass, const ConvArgs& a, hipStream_t s) {
+  const ConvGeom& g = p->g;
+  const int L = (pass == 0 || pass == 4) ? g.Lz : (pass == 2 ? g.Lx : g.Ly);
+  if (p->prec) launch_pass_L<double>(p, pass, L, a, s);
+  else launch_pass_L<float>(p, pass, L, a, s);
+  return PDDA_OK;
+}
+
+}  // namespace pdda

@@ -21,10 +21,24 @@ namespace pdda {
 constexpr int FFT_MAX_PASSES = 16;
 constexpr int FFT_NT = 256;  // threads per workgroup of every FFT kernel
 
+// Exact unsigned division by a divisor fixed for a launch: q = (x * m) >> 32 with
+// m = ceil(2^32 / d), exact whenever x * d < 2^32 (all tile and pass indices here are
+// below 2^13 and divisors below 2^13).
+struct FastDiv {
+  unsigned long long m;
+  __host__ __device__ static FastDiv make(unsigned d) {
+    FastDiv f;
+    f.m = ((1ull << 32) + d - 1) / d;
+    return f;
+  }
+  __device__ inline unsigned div(unsigned x) const { return (unsigned)((x * m) >> 32); }
+};
+
 struct FftPlan {
   int L;
   int npass;
   int radix[FFT_MAX_PASSES];
+  FastDiv ns_div[FFT_MAX_PASSES];  // division by Ns (product of the radices before pass p)
   int gen;  // 1 if a radix 7, 11 or 13 pass is present
 };
 
@@ -133,19 +147,22 @@ template <typename T> struct Dft<T, 8> {
 // One Stockham pass of radix R over the whole tile.
 // ---------------------------------------------------------------------------------
 template <typename T, int R, int EPT, bool INV>
-__device__ inline void stockham_pass(C<T>* data, int L, int ncol, int Ns, const C<T>* tw) {
+__device__ inline void stockham_pass(C<T>* data, int L, int ncol, FastDiv dcol, int Ns, FastDiv dns,
+                                     const C<T>* tw) {
   constexpr int MAXB = (EPT + R - 1) / R;
   const int LR = L / R;
   const int nb = LR * ncol;
   const int tstride = L / (Ns * R);
   C<T> v[MAXB][R];
+  int jj[MAXB], cc[MAXB];
 #pragma unroll
   for (int q = 0; q < MAXB; ++q) {
     const int b = threadIdx.x + q * FFT_NT;
+    jj[q] = (int)dcol.div((unsigned)b);
+    cc[q] = b - jj[q] * ncol;
     if (b < nb) {
-      const int j = b / ncol, col = b - j * ncol;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[q][r] = data[(j + r * LR) * ncol + col];
+      for (int r = 0; r < R; ++r) v[q][r] = data[(jj[q] + r * LR) * ncol + cc[q]];
     }
   }
   __syncthreads();
@@ -153,8 +170,9 @@ __device__ inline void stockham_pass(C<T>* data, int L, int ncol, int Ns, const 
   for (int q = 0; q < MAXB; ++q) {
     const int b = threadIdx.x + q * FFT_NT;
     if (b < nb) {
-      const int j = b / ncol, col = b - j * ncol;
-      const int k = j % Ns;
+      const int j = jj[q], col = cc[q];
+      const int jd = (int)dns.div((unsigned)j);
+      const int k = j - jd * Ns;
       if (INV) v[q][0] = cconj(v[q][0]);
 #pragma unroll
       for (int r = 1; r < R; ++r) {
@@ -162,7 +180,7 @@ __device__ inline void stockham_pass(C<T>* data, int L, int ncol, int Ns, const 
         v[q][r] = INV ? cmul(cconj(v[q][r]), w) : cmul(v[q][r], w);
       }
       Dft<T, R>::run(v[q], tw, L);
-      const int base = (j / Ns) * Ns * R + k;
+      const int base = jd * Ns * R + k;
 #pragma unroll
       for (int r = 0; r < R; ++r) data[(base + r * Ns) * ncol + col] = INV ? cconj(v[q][r]) : v[q][r];
     }
@@ -177,21 +195,23 @@ __device__ inline void stockham_pass(C<T>* data, int L, int ncol, int Ns, const 
 template <typename T, bool INV, bool GEN>
 __device__ inline void tile_fft(C<T>* data, int ncol, const FftPlan& pl, const C<T>* tw) {
   constexpr int EPT = TileCfg<T>::EPT;
+  const FastDiv dcol = FastDiv::make((unsigned)ncol);
   int Ns = 1;
   for (int p = 0; p < pl.npass; ++p) {
     const int R = pl.radix[p];
+    const FastDiv dns = pl.ns_div[p];
     switch (R) {
-      case 2: stockham_pass<T, 2, EPT, INV>(data, pl.L, ncol, Ns, tw); break;
-      case 3: stockham_pass<T, 3, EPT, INV>(data, pl.L, ncol, Ns, tw); break;
-      case 4: stockham_pass<T, 4, EPT, INV>(data, pl.L, ncol, Ns, tw); break;
-      case 5: stockham_pass<T, 5, EPT, INV>(data, pl.L, ncol, Ns, tw); break;
-      case 8: stockham_pass<T, 8, EPT, INV>(data, pl.L, ncol, Ns, tw); break;
+      case 2: stockham_pass<T, 2, EPT, INV>(data, pl.L, ncol, dcol, Ns, dns, tw); break;
+      case 3: stockham_pass<T, 3, EPT, INV>(data, pl.L, ncol, dcol, Ns, dns, tw); break;
+      case 4: stockham_pass<T, 4, EPT, INV>(data, pl.L, ncol, dcol, Ns, dns, tw); break;
+      case 5: stockham_pass<T, 5, EPT, INV>(data, pl.L, ncol, dcol, Ns, dns, tw); break;
+      case 8: stockham_pass<T, 8, EPT, INV>(data, pl.L, ncol, dcol, Ns, dns, tw); break;
       default:
         if constexpr (GEN) {
           switch (R) {
-            case 7: stockham_pass<T, 7, EPT, INV>(data, pl.L, ncol, Ns, tw); break;
-            case 11: stockham_pass<T, 11, EPT, INV>(data, pl.L, ncol, Ns, tw); break;
-            case 13: stockham_pass<T, 13, EPT, INV>(data, pl.L, ncol, Ns, tw); break;
+            case 7: stockham_pass<T, 7, EPT, INV>(data, pl.L, ncol, dcol, Ns, dns, tw); break;
+            case 11: stockham_pass<T, 11, EPT, INV>(data, pl.L, ncol, dcol, Ns, dns, tw); break;
+            case 13: stockham_pass<T, 13, EPT, INV>(data, pl.L, ncol, dcol, Ns, dns, tw); break;
             default: break;  // unreachable: plans are validated on the host
           }
         }

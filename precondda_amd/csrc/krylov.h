@@ -56,7 +56,7 @@ template <typename T> struct SolverT {
   int status(hipStream_t s);
   int result(void* out, hipStream_t s);
   int conv(ConvPlanBase* plan, int mode, const C<T>* x, C<T>* out, const C<T>* table, long long tstride, int dots,
-           const C<T>* d0, const C<T>* d1, int need, hipStream_t s);
+           const C<T>* d0, const C<T>* d1, int need, hipStream_t s, bool control = false);
 };
 
 // complex64 GPBiCG inner solves + FP64 true-residual refinement.

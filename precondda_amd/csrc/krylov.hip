@@ -15,13 +15,16 @@ constexpr int VNT = 256;  // threads per workgroup of the vector kernels
 // ---------------------------------------------------------------------------------
 // Reductions for vector kernels (same ticket scheme as the convolution epilogue).
 // ---------------------------------------------------------------------------------
-enum : int { FIN_STORE = 0, FIN_INIT = 1, FIN_NORM = 2 };
+enum : int { FIN_STORE = 0, FIN_INIT = 1, FIN_NORM = 2, FIN_CONTROL = 3 };
 
 __device__ inline void finalize_vec(SlotCtl& c, int kind, const double* re, const double* im, double p0,
                                     double p1, int ip0, int ip1) {
-  if (kind == FIN_STORE) {
+  if (kind == FIN_STORE || kind == FIN_CONTROL) {
     c.sums[0] = cd{re[0], im[0]};
     c.sums[1] = cd{re[1], im[1]};
+    // p0 != 0: the true-residual replacement of iteration 1 supplies this
+    // iteration's residual instead, and runs the control step itself.
+    if (kind == FIN_CONTROL && !(p0 != 0.0 && c.it == 1)) control_step(c);
   } else if (kind == FIN_INIT) {
     // p0 = target, p1 = ratio, ip0 = max_iter, ip1 = stagnation window
     const double n0 = sqrt(re[0]);
@@ -72,7 +75,10 @@ __global__ __launch_bounds__(VNT) void k_vec_finalize(const cd* __restrict__ par
                                                       int ip0, int ip1) {
   __shared__ double scratch[2 * ND * (VNT / 64)];
   const int slot = slot0 + blockIdx.x;
-  if (only_active && !(ctl[slot].flags & F_ACTIVE)) return;
+  if (only_active && !(ctl[slot].flags & F_ACTIVE)) {
+    if (kind == FIN_CONTROL && threadIdx.x == 0) ctl[slot].flags &= ~(F_RESET | F_RECOMPUTE | F_IMPROVED);
+    return;
+  }
   const cd* p = partial + (size_t)slot * nwg * 6;
   double re[ND], im[ND];
   for (int d = 0; d < ND; ++d) { re[d] = 0.0; im[d] = 0.0; }
@@ -186,84 +192,23 @@ __global__ void k_norm_from_sums(SlotCtl* ctl, int nslots) {
   if (s < nslots && (ctl[s].flags & F_ACTIVE)) ctl[s].current_norm = sqrt(ctl[s].sums[0].x);
 }
 
-// Per-slot control step: norm checks, best-iterate flag, stop, rho update, reset decision.
-__global__ void k_control(SlotCtl* ctl, int nslots) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nslots) return;
-  SlotCtl& c = ctl[s];
-  c.flags &= ~(F_RESET | F_RECOMPUTE | F_IMPROVED);
-  if (!(c.flags & F_ACTIVE)) return;
-  const int i = c.it;
-  const double cur = sqrt(c.sums[0].x);
-  const cd rdot = c.sums[1];
-  c.current_norm = cur;
-  if (i == 0) c.first_norm = cur;
-  c.last_change += 1;
-  if (!isnan(cur) && (isnan(c.lowest_norm) || cur < c.lowest_norm)) {
-    c.last_change = 0;
-    c.lowest_norm = cur;
-    c.flags |= F_IMPROVED;
-  }
-  if (isnan(cur) || c.last_change >= c.stagnation) {
-    c.flags &= ~F_ACTIVE;
-    c.status = ST_FAILED;
-    c.stop_it = i;
-    return;
-  }
-  if (cur < c.target * c.initial_norm) {
-    c.flags &= ~F_ACTIVE;
-    c.status = ST_CONVERGED;
-    c.stop_it = i;
-    return;
-  }
-  c.rho0 = c.rho1;
-  c.rho1 = rdot;
-  bool reset = false;
-  if (c.ratio != 0.0) {
-    if (i == 2 || (i > 2 && cur < c.previous_lowest / c.ratio)) {
-      reset = true;
-      c.previous_lowest = cur;
-      c.last_change = 0;
-    }
-  }
-  if (reset) {
-    c.flags |= F_RESET | F_WAS_RESET;
-    if (i % 100 != 0) c.flags |= F_RECOMPUTE;
-    c.beta = cd{0.0, 0.0};
-    c.resets += 1;
-  } else {
-    c.flags &= ~F_WAS_RESET;
-    c.beta = cd_div(cd_mul(cd_div(c.alpha, c.zeta), c.rho1), c.rho0);
-  }
-  c.it = i + 1;
-  if (c.it >= c.max_iter) {
-    c.flags &= ~(F_ACTIVE | F_RESET | F_RECOMPUTE);
-    c.status = ST_MAXITER;
-    c.stop_it = i;
-  }
-}
-
-template <typename T>
-__global__ __launch_bounds__(VNT) void k_copy_best(const C<T>* __restrict__ x, C<T>* __restrict__ best, long long n3,
-                                                   const SlotCtl* ctl, int nwg) {
-  const int slot = blockIdx.y;
-  if (!(ctl[slot].flags & F_IMPROVED)) return;
-  const long long off = (long long)slot * n3;
-  for (long long i = blockIdx.x * (long long)VNT + threadIdx.x; i < n3; i += (long long)nwg * VNT)
-    best[off + i] = x[off + i];
-}
-
+// best = x when this iteration improved the residual (also on the iteration that
+// stops the slot); then for running slots
 // reset: rhat = r; w = 0; p = r        else: w = t + beta v; p = r + beta (p - u)
 template <typename T>
 __global__ __launch_bounds__(VNT) void k_upd3(VecSet<T> v, long long n3, const SlotCtl* ctl, int nwg) {
   const int slot = blockIdx.y;
   const SlotCtl& c = ctl[slot];
-  if (!(c.flags & F_ACTIVE)) return;
+  const bool improved = (c.flags & F_IMPROVED) != 0;
+  const bool active = (c.flags & F_ACTIVE) != 0;
+  if (!improved && !active) return;
   const bool reset = (c.flags & F_RESET) != 0;
   const cd be = c.beta;
   const long long off = (long long)slot * n3;
   for (long long i = blockIdx.x * (long long)VNT + threadIdx.x; i < n3; i += (long long)nwg * VNT) {
     const long long j = off + i;
+    if (improved) v.best[j] = v.x[j];
+    if (!active) continue;
     const C<T> rv = v.r[j];
     if (reset) {
       v.rhat[j] = rv;
@@ -405,8 +350,9 @@ int SolverT<T>::load(int slot, const void* E, double target, int max_iter, doubl
 
 template <typename T>
 int SolverT<T>::conv(ConvPlanBase* plan, int mode, const C<T>* x, C<T>* out, const C<T>* table, long long tstride,
-                     int dots, const C<T>* d0, const C<T>* d1, int need, hipStream_t s) {
+                     int dots, const C<T>* d0, const C<T>* d1, int need, hipStream_t s, bool control) {
   ConvArgs a;
+  a.fin_control = control;
   a.mode = mode;
   a.x = x;
   a.out = out;
@@ -449,21 +395,22 @@ int SolverT<T>::iterate(int n_iter, hipStream_t s) {
     if ((rc = conv(&mv, MODE_INTERACT, sh, vs.t, ghat, ghat_stride, DOT_ZETA, vs.y, s_new, NEED_ACTIVE, s)))
       return rc;
     hipLaunchKernelGGL(k_upd2<T>, vg, dim3(VNT), 0, s, vs, s_new, s_prev, vs.y, n3, partial, counters, ctl, nwg);
+    const bool replace = !lockstep || host_iter == 1;
     hipLaunchKernelGGL(k_vec_finalize<2>, dim3(nslots), dim3(VNT), 0, s, (const cd*)partial, nwg, ctl, 0,
-                       (int)FIN_STORE, 1, 0.0, 0.0, 0, 0);
-    // true-residual replacement at the slot's iteration 1
-    if (!lockstep || host_iter == 1) {
+                       (int)FIN_CONTROL, 1, replace ? 1.0 : 0.0, 0.0, 0, 0);
+    // true-residual replacement at the slot's iteration 1 (its finalize runs the
+    // control step for those slots)
+    if (replace) {
       const C<T>* xs = vs.x;
       if (precon) {
         if ((rc = conv(&pc, MODE_PRECOND, vs.x, vs.phat, mhat, mhat_stride, DOT_NONE, nullptr, nullptr, NEED_REPLACE, s)))
           return rc;
         xs = vs.phat;
       }
-      if ((rc = conv(&mv, MODE_RESIDUAL, xs, vs.r, ghat, ghat_stride, DOT_RESID, vs.rhat, nullptr, NEED_REPLACE, s)))
+      if ((rc = conv(&mv, MODE_RESIDUAL, xs, vs.r, ghat, ghat_stride, DOT_RESID, vs.rhat, nullptr, NEED_REPLACE, s,
+                     true)))
         return rc;
     }
-    hipLaunchKernelGGL(k_control, dim3((nslots + 63) / 64), dim3(64), 0, s, ctl, nslots);
-    hipLaunchKernelGGL(k_copy_best<T>, vg, dim3(VNT), 0, s, vs.x, vs.best, n3, ctl, nwg);
     // reset with residual recompute
     {
       const C<T>* xs = vs.x;

@@ -51,6 +51,64 @@ struct SlotCtl {
   int flags, status, stop_it, resets;
 };
 
+// The per-iteration control step of the reference loop (Gpbicgstab.py:1076-1158), run
+// on one thread per slot once the iteration's residual sums are in c.sums[0..1]:
+// norm, best-iterate flag, NaN/stagnation and convergence exits, rho update, and the
+// reset decision (forced at iteration 2, then whenever the norm drops below
+// previous_lowest / ratio; a reset recomputes r unless i % 100 == 0).
+__device__ inline void control_step(SlotCtl& c) {
+  c.flags &= ~(F_RESET | F_RECOMPUTE | F_IMPROVED);
+  if (!(c.flags & F_ACTIVE)) return;
+  const int i = c.it;
+  const double cur = sqrt(c.sums[0].x);
+  const cd rdot = c.sums[1];
+  c.current_norm = cur;
+  if (i == 0) c.first_norm = cur;
+  c.last_change += 1;
+  if (!isnan(cur) && (isnan(c.lowest_norm) || cur < c.lowest_norm)) {
+    c.last_change = 0;
+    c.lowest_norm = cur;
+    c.flags |= F_IMPROVED;
+  }
+  if (isnan(cur) || c.last_change >= c.stagnation) {
+    c.flags &= ~F_ACTIVE;
+    c.status = ST_FAILED;
+    c.stop_it = i;
+    return;
+  }
+  if (cur < c.target * c.initial_norm) {
+    c.flags &= ~F_ACTIVE;
+    c.status = ST_CONVERGED;
+    c.stop_it = i;
+    return;
+  }
+  c.rho0 = c.rho1;
+  c.rho1 = rdot;
+  bool reset = false;
+  if (c.ratio != 0.0) {
+    if (i == 2 || (i > 2 && cur < c.previous_lowest / c.ratio)) {
+      reset = true;
+      c.previous_lowest = cur;
+      c.last_change = 0;
+    }
+  }
+  if (reset) {
+    c.flags |= F_RESET | F_WAS_RESET;
+    if (i % 100 != 0) c.flags |= F_RECOMPUTE;
+    c.beta = cd{0.0, 0.0};
+    c.resets += 1;
+  } else {
+    c.flags &= ~F_WAS_RESET;
+    c.beta = cd_div(cd_mul(cd_div(c.alpha, c.zeta), c.rho1), c.rho0);
+  }
+  c.it = i + 1;
+  if (c.it >= c.max_iter) {
+    c.flags &= ~(F_ACTIVE | F_RESET | F_RECOMPUTE);
+    c.status = ST_MAXITER;
+    c.stop_it = i;
+  }
+}
+
 __device__ inline bool slot_wanted(const SlotCtl* ctl, int slot, int need) {
   if (ctl == nullptr || need == NEED_ALL) return true;
   const SlotCtl& c = ctl[slot];
