@@ -30,6 +30,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--profile-matvec", action="store_true",
+                    help="only run the matvec kernels (complex64 and complex128, 20 each) and one 256 MiB "
+                         "device copy, for rocprofv3 counter collection")
     ap.add_argument("--precon", action="store_true",
                     help="use the damped circulant preconditioner (slower on this workload: each apply costs a "
                          "matvec and saves fewer than half the iterations)")
@@ -79,6 +82,23 @@ def main():
     torch.cuda.synchronize()
     setup_s = time.perf_counter() - t0
     log(f"setup {setup_s:.2f}s grid {wl.grid} dipoles {wl.n_dipoles} k {wl.k:.5f}")
+
+    if args.profile_matvec:
+        # calibration copy of known size (256 MiB read + 256 MiB written)
+        src = torch.empty(32 * 1024 * 1024, dtype=torch.complex64, device=dev)
+        dst = torch.empty_like(src)
+        src.real.fill_(1.0)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        for dt, G, ia in ((C64, G32, ia32), (C128, G64, ia64)):
+            plan = conv_plan(1 if dt == C128 else 0, wl.grid, (nx, ny, nz))
+            x = E64[0].to(dt).contiguous()
+            out = torch.empty_like(x)
+            for _ in range(20):
+                plan.apply(_lib.PDDA_MODE_INTERACT, x, out, G, inv_alpha=ia, mask=mask)
+        torch.cuda.synchronize()
+        log("profile-matvec done")
+        return
 
     # ---------------- mixed precision (the measured path)
     ms = MixedSolver(wl.grid, nrhs, args.precon)

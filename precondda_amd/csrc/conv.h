@@ -88,6 +88,9 @@ struct ConvArgs {
 // the passes whose FFT length has a compile-time plan; launches one pass.
 void fast_setup(ConvPlanBase* p);
 int fast_launch(ConvPlanBase* p, int pass, const ConvArgs& a, hipStream_t s);
+// Times the candidate tile shapes of every fast pass on the device (on a private
+// stream, with plan-owned scratch) and keeps the fastest. PDDA_NO_AUTOTUNE=1 skips it.
+int fast_autotune(ConvPlanBase* p);
 
 // Host API (conv.hip).
 bool fft_length_supported(int L);

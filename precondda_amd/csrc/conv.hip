@@ -522,9 +522,11 @@ int conv_plan_init(ConvPlanBase* p, int prec, int nx, int ny, int nz, int hx, in
   const int tiles_y = ceil_div(ny, g.rows1);
   p->nwg5 = nx * tiles_y;
   fast_setup(p);  // may switch P5 to a fast tile (and so change nwg5)
-  PDDA_HIP_CHECK(hipMalloc(&p->partial, (size_t)nslots * std::max(p->nwg5, nx * tiles_y) * 6 * sizeof(cd)));
+  // partials sized for the smallest P5 tile (one row), the autotuner may pick any
+  PDDA_HIP_CHECK(hipMalloc(&p->partial, (size_t)nslots * nx * ny * 6 * sizeof(cd)));
   PDDA_HIP_CHECK(hipMalloc(&p->counters, (size_t)nslots * sizeof(int)));
   PDDA_HIP_CHECK(hipMemset(p->counters, 0, (size_t)nslots * sizeof(int)));
+  if ((rc = fast_autotune(p))) return rc;
   p->lds[0] = (size_t)(g.Lz + g.Lz * 3 * g.rows1) * es;
   p->lds[1] = (size_t)(g.Ly + g.Ly * g.ncol2) * es;
   p->lds[2] = (size_t)(g.Lx + g.Lx * 6 * g.wz3) * es;

@@ -5,6 +5,8 @@
 //   sides to keep the HBM accesses contiguous.
 #include <algorithm>
 #include <cstdlib>
+#include <string>
+#include <vector>
 
 #include "conv.h"
 #include "fft_ct.h"
@@ -93,11 +95,17 @@ __global__ __launch_bounds__(FFT_NT) void kf_xconv(C<T>* __restrict__ B, const C
   constexpr int hx = L / 2;
   const int hy = g.Ly / 2, hz = g.Lz / 2;
   const int nkzb = (g.Lz + wz3 - 1) / wz3;
-  const int t = blockIdx.x;
-  const int per_slot = (hy + 1) * nkzb;
-  const int slot = t / per_slot;
-  const int rem = t - slot * per_slot;
-  const int qy = rem / nkzb, kb = rem - qy * nkzb;
+  // XCD-aware tile order: workgroups are dispatched round-robin over the 8 XCDs, so
+  // XCD x runs blocks x, x+8, ...; give each XCD a contiguous range of logical tiles
+  // ordered (ky pair, kz block interleaved with its mirror block, slot) so that tiles
+  // reading the same reduced-table rows run back to back on one L2.
+  const int nblk = gridDim.x, b = blockIdx.x, xcd = b & 7, r = b >> 3;
+  const int per_xcd = nblk >> 3, remx = nblk & 7;
+  const int lt = xcd * per_xcd + min(xcd, remx) + r;
+  const int slot = lt % g.nslots;
+  const int tq = lt / g.nslots;
+  const int qy = tq / nkzb, kbp = tq - qy * nkzb;
+  const int kb = (kbp & 1) ? nkzb - 1 - (kbp >> 1) : (kbp >> 1);
   if (!slot_wanted(ctl, slot, need)) return;
   const int nyr = (qy == 0 || qy == hy) ? 1 : 2;
   const int kz0 = kb * wz3, wz = min(wz3, g.Lz - kz0);
@@ -120,20 +128,38 @@ __global__ __launch_bounds__(FFT_NT) void kf_xconv(C<T>* __restrict__ B, const C
     return (pos < nx && ok) ? base[ad] : zero<T>();
   };
   auto stl = [&](int pos, int col, C<T> v) { lds[pos * nc + col] = v; };
-  ct_fft<T, L, false, false>(cm, tw, lds, ld0, stl);
-  __syncthreads();
-
+  // reduced-table entries of this tile, (hx+1) x wz, fetched before the forward FFT so
+  // that their latency overlaps it (up to two entries per thread held in registers)
   const C<T>* tab = table + slot * table_stride + (long long)qy * (hz + 1) * 6;
   const long long mxs = (long long)(hy + 1) * (hz + 1) * 6;
+  auto gaddr_tab = [&](int e) {
+    const int mx = e / wz, q = e - mx * wz, kz = kz0 + q;
+    const int mz = kz > hz ? g.Lz - kz : kz;
+    return tab + mx * mxs + mz * 6;
+  };
+  C<T> gpre[2][6];
+  {
+    const int ne = (hx + 1) * wz;
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int e = threadIdx.x + u * FFT_NT;
+      if (e < ne) {
+        const C<T>* gp = gaddr_tab(e);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) gpre[u][c] = gp[c];
+      }
+    }
+  }
+  ct_fft<T, L, false, false>(cm, tw, lds, ld0, stl);
+
+  __syncthreads();
   const int nent = (hx + 1) * wz;
   const FastDiv dwz = FastDiv::make((unsigned)wz);
-  for (int e = threadIdx.x; e < nent; e += FFT_NT) {
+  auto apply = [&](int e, const C<T> (&gq)[6]) {
     const int mx = (int)dwz.div((unsigned)e), q = e - mx * wz, kz = kz0 + q;
     const bool fz = kz > hz;
-    const int mz = fz ? g.Lz - kz : kz;
-    const C<T>* gp = tab + mx * mxs + mz * 6;
-    const C<T> g0 = cscale(gp[0], scale), g1 = cscale(gp[1], scale), g2 = cscale(gp[2], scale);
-    const C<T> g3 = cscale(gp[3], scale), g4 = cscale(gp[4], scale), g5 = cscale(gp[5], scale);
+    const C<T> g0 = cscale(gq[0], scale), g1 = cscale(gq[1], scale), g2 = cscale(gq[2], scale);
+    const C<T> g3 = cscale(gq[3], scale), g4 = cscale(gq[4], scale), g5 = cscale(gq[5], scale);
     const int nkx = (mx == 0 || mx == hx) ? 1 : 2;
     for (int ix = 0; ix < nkx; ++ix) {
       const int kx = ix ? L - mx : mx;
@@ -149,6 +175,18 @@ __global__ __launch_bounds__(FFT_NT) void kf_xconv(C<T>* __restrict__ B, const C
         dp[2] = cadd(cadd(cmul(h2, x0), cmul(h4, x1)), cmul(g5, x2));
       }
     }
+  };
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int e = threadIdx.x + u * FFT_NT;
+    if (e < nent) apply(e, gpre[u]);
+  }
+  for (int e = threadIdx.x + 2 * FFT_NT; e < nent; e += FFT_NT) {
+    C<T> gq[6];
+    const C<T>* gp = gaddr_tab(e);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) gq[c] = gp[c];
+    apply(e, gq);
   }
   __syncthreads();
   auto ldl = [&](int pos, int col) { return lds[pos * nc + col]; };
@@ -407,6 +445,110 @@ void fast_setup(ConvPlanBase* p) {
     }
   }
   if (p->fast[0]) p->nwg5 = g.nx * ((g.ny + p->f_rows - 1) / p->f_rows);
+}
+
+namespace {
+
+std::vector<int> pow2_candidates(int L, int prec, int c3, int pmax) {
+  std::vector<int> out;
+  const int lim = pick_pow2(L, prec, c3, 1 << 20, pmax);
+  for (int q = 1; q <= lim; q *= 2) out.push_back(q);
+  return out;
+}
+
+// Mean milliseconds of `reps` launches of the given passes.
+float time_passes(ConvPlanBase* p, const int* passes, int np, const ConvArgs& a, hipStream_t s, hipEvent_t e0,
+                  hipEvent_t e1, int reps) {
+  for (int q = 0; q < np; ++q) fast_launch(p, passes[q], a, s);  // warm-up
+  (void)hipEventRecord(e0, s);
+  for (int r = 0; r < reps; ++r)
+    for (int q = 0; q < np; ++q) fast_launch(p, passes[q], a, s);
+  (void)hipEventRecord(e1, s);
+  (void)hipEventSynchronize(e1);
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  return ms / reps;
+}
+
+}  // namespace
+
+int fast_autotune(ConvPlanBase* p) {
+  if (!(p->fast[0] || p->fast[1] || p->fast[2])) return PDDA_OK;
+  if (const char* e = getenv("PDDA_NO_AUTOTUNE"); e && e[0] == '1') return PDDA_OK;
+  ConvGeom& g = p->g;
+  const size_t es = p->prec ? 16 : 8;
+  hipStream_t s;
+  PDDA_HIP_CHECK(hipStreamCreate(&s));
+  hipEvent_t e0, e1;
+  PDDA_HIP_CHECK(hipEventCreate(&e0));
+  PDDA_HIP_CHECK(hipEventCreate(&e1));
+  const size_t tab_elems = (size_t)(g.Lx / 2 + 1) * (g.Ly / 2 + 1) * (g.Lz / 2 + 1) * 6;
+  void* tab = nullptr;
+  PDDA_HIP_CHECK(hipMalloc(&tab, tab_elems * es));
+  PDDA_HIP_CHECK(hipMemsetAsync(tab, 0, tab_elems * es, s));
+  PDDA_HIP_CHECK(hipMemsetAsync(p->work_a, 0, (size_t)g.a_stride * g.nslots * es, s));
+  PDDA_HIP_CHECK(hipMemsetAsync(p->work_b, 0, (size_t)g.b_stride * g.nslots * es, s));
+  ConvArgs a;
+  a.mode = MODE_FIELD;
+  a.x = p->work_b;    // any 3N-per-slot buffer works as timing input
+  a.out = p->work_b;  // P5 writes box vectors here while it reads work_a
+  a.table = tab;
+  const int reps = 4;
+  if (p->fast[0]) {
+    int pmax = 1;
+    while (pmax * 2 <= g.ny) pmax *= 2;
+    const int passes[2] = {0, 4};
+    float best = 1e30f;
+    int bestr = p->f_rows;
+    for (int r : pow2_candidates(g.Lz, p->prec, 3, pmax)) {
+      p->f_rows = r;
+      p->f_lds[0] = p->f_lds[4] = (size_t)(g.Lz + (size_t)g.Lz * 3 * r) * es;
+      p->nwg5 = g.nx * ((g.ny + r - 1) / r);
+      const float t = time_passes(p, passes, 2, a, s, e0, e1, reps);
+      if (t < best) { best = t; bestr = r; }
+    }
+    p->f_rows = bestr;
+    p->f_lds[0] = p->f_lds[4] = (size_t)(g.Lz + (size_t)g.Lz * 3 * bestr) * es;
+    p->nwg5 = g.nx * ((g.ny + bestr - 1) / bestr);
+  }
+  if (p->fast[1]) {
+    int pmax = 1;
+    while (pmax * 2 <= 3 * g.Lz) pmax *= 2;
+    const int passes[2] = {1, 3};
+    float best = 1e30f;
+    int bestc = p->f_ncol;
+    for (int c : pow2_candidates(g.Ly, p->prec, 1, pmax)) {
+      if (c < 4) continue;
+      p->f_ncol = c;
+      p->f_lds[1] = p->f_lds[3] = (size_t)(g.Ly + (size_t)g.Ly * c) * es;
+      const float t = time_passes(p, passes, 2, a, s, e0, e1, reps);
+      if (t < best) { best = t; bestc = c; }
+    }
+    p->f_ncol = bestc;
+    p->f_lds[1] = p->f_lds[3] = (size_t)(g.Ly + (size_t)g.Ly * bestc) * es;
+  }
+  if (p->fast[2]) {
+    int pmax = 1;
+    while (pmax * 2 <= g.Lz) pmax *= 2;
+    const int passes[1] = {2};
+    float best = 1e30f;
+    int bestw = p->f_wz;
+    for (int w : pow2_candidates(g.Lx, p->prec, 6, pmax)) {
+      p->f_wz = w;
+      p->f_lds[2] = (size_t)(g.Lx + (size_t)g.Lx * 6 * w) * es;
+      const float t = time_passes(p, passes, 1, a, s, e0, e1, reps);
+      if (t < best) { best = t; bestw = w; }
+    }
+    p->f_wz = bestw;
+    p->f_lds[2] = (size_t)(g.Lx + (size_t)g.Lx * 6 * bestw) * es;
+  }
+  hipError_t err = hipStreamSynchronize(s);
+  (void)hipFree(tab);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipStreamDestroy(s);
+  if (err != hipSuccess) { set_error(std::string("autotune: ") + hipGetErrorString(err)); return PDDA_ERR_HIP; }
+  return PDDA_OK;
 }
 
 int fast_launch(ConvPlanBase* p, int pass, const ConvArgs& a, hipStream_t s) {
