@@ -26,6 +26,12 @@ import numpy as np
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", default="precision", choices=["precision", "batching", "large", "plain-small"],
+                    help="precision (default, the metric's configuration): 64^3 sphere, 2 RHS; batching: "
+                         "48x48x96 spheroid, 64 directions x 2 polarisations solved as one block; large: porous "
+                         "256x256x128 cuboid, 2 RHS; plain-small: 16^3 sphere, 1 RHS")
+    ap.add_argument("--plain-steps", type=int, default=None,
+                    help="timed FP64 plain solves (default: min(steps, 3))")
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -65,7 +71,8 @@ def main():
     dev = torch.device("cuda", 0)
     log("building workload")
     t0 = time.perf_counter()
-    wl = workloads.precision()
+    wl = {"precision": workloads.precision, "batching": workloads.batching, "large": workloads.porous_cuboid,
+          "plain-small": workloads.sphere_small}[args.workload]()
     nx, ny, nz = wl.grid
     N = nx * ny * nz
     Np = (nx + 1) * (ny + 1) * (nz + 1)
@@ -137,7 +144,8 @@ def main():
     log("fp64 warmup done")
     torch.cuda.synchronize()
     ptimes = []
-    for _ in range(max(1, min(args.steps, 3))):
+    n_plain = args.plain_steps if args.plain_steps is not None else max(1, min(args.steps, 3))
+    for _ in range(n_plain):
         torch.cuda.synchronize()
         t = time.perf_counter()
         Pp, st = plain_step()
@@ -189,6 +197,18 @@ def main():
                     "GBps": round(sum(byts) / t_mv / 1e9, 1), "frac_of_peak": round(sum(byts) / t_mv / peak, 4),
                     "bytes": int(sum(byts)),
                     "pass_GBps": [round(bb / (m * 1e-3) / 1e9, 1) for bb, m in zip(byts, ms5)]}
+    if nrhs > 1:
+        # the whole block of right-hand sides in one launch sequence (shared table)
+        plan = conv_plan(0, wl.grid, (nx, ny, nz), nrhs)
+        xb = E32.contiguous()
+        outb = torch.empty_like(xb)
+        plan.time_passes(_lib.PDDA_MODE_INTERACT, xb, outb, G32, ia32, mask, reps=2)
+        ms5 = plan.time_passes(_lib.PDDA_MODE_INTERACT, xb, outb, G32, ia32, mask, reps=10)
+        bb = (82 * N * nrhs + 6 * Np) * 8
+        t_mv = sum(ms5) * 1e-3
+        mv["c64_block"] = {"rhs": nrhs, "ms_per_pass": [round(v, 5) for v in ms5], "ms": round(sum(ms5), 5),
+                           "GBps": round(bb / t_mv / 1e9, 1), "frac_of_peak": round(bb / t_mv / peak, 4),
+                           "bytes": int(bb)}
     log(f"matvec {mv}")
     dom = int(np.argmax(mv["c64"]["ms_per_pass"]))
     names = ["k_zfwd", "k_axis(y-fwd)", "k_xconv", "k_axis(y-inv)", "k_zinv"]
