@@ -26,10 +26,11 @@ import numpy as np
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="precision", choices=["precision", "batching", "large", "plain-small"],
+    ap.add_argument("--workload", default="precision", choices=["precision", "batching", "large", "plain-small", "serving"],
                     help="precision (default, the metric's configuration): 64^3 sphere, 2 RHS; batching: "
                          "48x48x96 spheroid, 64 directions x 2 polarisations solved as one block; large: porous "
-                         "256x256x128 cuboid, 2 RHS; plain-small: 16^3 sphere, 1 RHS")
+                         "256x256x128 cuboid, 2 RHS; plain-small: 16^3 sphere, 1 RHS; serving: Poisson stream of "
+                         "200 uneven targets through the continuous-batching service")
     ap.add_argument("--plain-steps", type=int, default=None,
                     help="timed FP64 plain solves (default: min(steps, 3))")
     ap.add_argument("--steps", type=int, default=5)
@@ -39,6 +40,12 @@ def parse():
     ap.add_argument("--profile-matvec", action="store_true",
                     help="only run the matvec kernels (complex64 and complex128, 20 each) and one 256 MiB "
                          "device copy, for rocprofv3 counter collection")
+    ap.add_argument("--rate", type=float, default=40.0, help="serving: Poisson arrival rate (requests/s)")
+    ap.add_argument("--requests", type=int, default=200, help="serving: number of requests")
+    ap.add_argument("--slots", type=int, default=16, help="serving: slots per padded-box bin")
+    ap.add_argument("--warm-requests", type=int, default=40,
+                    help="serving: requests (another seed) served before the timed stream")
+    ap.add_argument("--hbm-budget", type=float, default=128e9, help="serving: HBM budget for bins (bytes)")
     ap.add_argument("--precon", action="store_true",
                     help="use the damped circulant preconditioner (slower on this workload: each apply costs a "
                          "matvec and saves fewer than half the iterations)")
@@ -69,6 +76,8 @@ def main():
 
     torch.cuda.init()
     dev = torch.device("cuda", 0)
+    if args.workload == "serving":
+        return serving_main(args)
     log("building workload")
     t0 = time.perf_counter()
     wl = {"precision": workloads.precision, "batching": workloads.batching, "large": workloads.porous_cuboid,
@@ -271,6 +280,66 @@ def main():
         "mixed": {"outer": [i.outer for i in infos], "inner_iters": [i.inner_iters for i in infos],
                   "final_rel": [i.final_rel for i in infos]},
         "accuracy": acc, "matvec": mv, "roofline": roofline, "cpu_baseline": cpu, "setup_s": round(setup_s, 3),
+    }
+    print(json.dumps(line), flush=True)
+
+
+def serving_main(args):
+    """Continuous batching over the seeded request stream: value = requests / wall time
+    from the first arrival to the last result; latency = arrival to result-ready."""
+    import torch
+
+    from precondda_amd.serving import ServingEngine, latency_stats, make_requests, padded_size
+
+    def engine():
+        return ServingEngine(hbm_budget=args.hbm_budget, slots_per_bin=args.slots, tol=args.tol, check_every=8)
+
+    reqs = make_requests(args.requests, seed=0, rate=args.rate)
+    eng = engine()
+    for w in range(args.warmup):
+        # a service that has been running holds bins for the common boxes
+        warm = make_requests(args.warm_requests, seed=1000 + w)
+        eng.run(warm)
+        log(f"serving warmup {w} done ({len(eng.bins)} bins, {eng.used_bytes() / 1e9:.1f} GB)")
+    torch.cuda.synchronize()
+    eng.bin_create_s = 0.0
+    n_bins0 = len(eng.bins)
+    res, wall = eng.run(reqs, log=log)
+    torch.cuda.synchronize()
+    st = latency_stats(res)
+    n_ok = sum(r.ok for r in res)
+    bins = {}
+    for r in res:
+        key = "x".join(str(v) for v in r.box)
+        e = bins.setdefault(key, {"n": 0, "inner": 0.0, "service_ms": 0.0, "queue_ms": 0.0})
+        e["n"] += 1
+        e["inner"] += r.inner_iters
+        e["service_ms"] += 1e3 * (r.done - r.admitted)
+        e["queue_ms"] += 1e3 * (r.admitted - r.arrival)
+    for e in bins.values():
+        for f in ("inner", "service_ms", "queue_ms"):
+            e[f] = round(e[f] / e["n"], 2)
+    qs = sorted(1e3 * (r.admitted - r.arrival) for r in res)
+    worst = max(r.rel_residual for r in res)
+    log(f"serving: {len(res)} results in {wall:.2f}s, {n_ok} converged, latency {st}")
+    line = {
+        "metric": "DDA solves/s & p99 latency, mixed-prec vs FP64 plain @1e-6 resid; matvec GB/s",
+        "value": round(len(res) / wall, 4), "unit": "solves/s", "n_gpus": 1, "steps": len(res), "warmup": args.warmup,
+        "ms_per_step": round(1e3 * wall / len(res), 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "complex64 bulk + complex128 refinement", "data": "synthetic",
+        "config": {"workload": f"serving: {args.requests} requests (spheres/spheroids/cuboids, long edge in "
+                               f"{{24,32,48,64,96}}), Poisson arrivals at {args.rate}/s, seed 0, continuous batching "
+                               f"in padded-box bins of {args.slots} slots, mixed complex64 GPBiCG + FP64 refinement "
+                               f"to FP64 relative residual {args.tol:.0e}",
+                   "parallelism": "single GPU"},
+        "latency_ms": {k: round(v, 3) for k, v in st.items()},
+        "serving": {"wall_s": round(wall, 3), "converged": n_ok, "worst_rel_residual": worst,
+                    "bins": bins,
+                    "queue_ms_p50": round(qs[len(qs) // 2], 3), "queue_ms_max": round(qs[-1], 3),
+                    "bins_at_start": n_bins0, "bins_created": len(eng.bins) - n_bins0,
+                    "bin_create_s": round(eng.bin_create_s, 3), "hbm_gb": round(eng.used_bytes() / 1e9, 2), "inner_iters_mean": float(np.mean([r.inner_iters for r in res])),
+                    "outer_mean": float(np.mean([r.outer for r in res]))},
+        "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
 

@@ -76,6 +76,15 @@ int pdda_fft_length_supported(int L);
  * out_c128 (required) and out_c64 (optional) hold (nx+1)(ny+1)(nz+1)*6 elements. */
 int pdda_green_build(int nx, int ny, int nz, double k, double d, void* out_c128, void* out_c64, void* stream);
 
+/* The same table build with its FP64 FFT workspace and twiddles held by a plan, so that
+ * repeated builds for one box (a different k per target) allocate nothing and do not
+ * synchronise. Builds of one plan must be ordered on one stream. Same citation. */
+typedef struct pdda_green_plan pdda_green_plan;
+int pdda_green_plan_create(pdda_green_plan** out, int nx, int ny, int nz);
+int pdda_green_plan_build(pdda_green_plan* g, double k, double d, void* out_c128, void* out_c64, void* stream);
+long long pdda_green_plan_bytes(const pdda_green_plan* g);
+int pdda_green_plan_destroy(pdda_green_plan* g);
+
 /* Circulant preconditioner table M(q) = (gamma I - G(q))^-1 from a complex128 reduced
  * table on the (2hx,2hy,2hz) grid. Takes the place of the absent
  * pre_preconditioner/circulant_approximation (ddaforgit.py:1118-1136). */
@@ -152,6 +161,9 @@ int pdda_solver_iterate(pdda_solver* s, int n_iter, void* stream);
 int pdda_solver_status(pdda_solver* s, pdda_slot_status* out, int nslots, void* stream);
 /* P = M x (converged) or M x_best (otherwise) for every slot (Gpbicgstab.py:1162-1201). */
 int pdda_solver_result(pdda_solver* s, void* P_out, void* stream);
+/* pdda_solver_iterate replayed from a captured hipGraph (n_iter even; captured on
+ * first use, re-captured when n_iter or the stream changes). Same results. */
+int pdda_solver_iterate_graph(pdda_solver* s, int n_iter, void* stream);
 
 /* complex64 GPBiCG inner solves with FP64 true-residual refinement. */
 int pdda_mixed_create(pdda_mixed** out, int nx, int ny, int nz, int px, int py, int pz, int nslots, int precon);
@@ -164,6 +176,26 @@ int pdda_mixed_solve(pdda_mixed* m, const void* E_inc_c128, double tol, double i
                      int max_inner, double ratio, int stagnation, int check_every, void* stream);
 /* Copies the FP64 solutions (nslots box vectors, complex128) and per-slot reports. */
 int pdda_mixed_result(pdda_mixed* m, void* P_out_c128, pdda_mixed_report* rep, int nslots, void* stream);
+
+/* Serving (continuous batching): slots with independent lifecycles. No counterpart in
+ * the reference. set_serving(1) lets slots start at different iterations. */
+int pdda_mixed_set_serving(pdda_mixed* m, int on);
+/* Starts slot `slot` on E_inc (3N complex128 of that slot): b = E*mask, x = 0, r = b,
+ * and loads the complex64 inner solve with r/||r||. */
+int pdda_mixed_slot_begin(pdda_mixed* m, int slot, const void* E64_slot, double inner_target, int max_inner,
+                          double ratio, int stagnation, void* stream);
+/* Reloads the inner solve of `slot` with the current r/||r|| (after a refinement). */
+int pdda_mixed_slot_reload(pdda_mixed* m, int slot, double inner_target, int max_inner, double ratio,
+                           int stagnation, void* stream);
+/* n_iter inner iterations for every running slot, eagerly or from a captured graph. */
+int pdda_mixed_iterate(pdda_mixed* m, int n_iter, int use_graph, void* stream);
+/* Synchronises and reports the inner (complex64) slot states. */
+int pdda_mixed_inner_status(pdda_mixed* m, pdda_slot_status* out, int nslots, void* stream);
+/* FP64 refinement of the slots with sel_host[q] != 0: x += ||r|| d, r = b - A x;
+ * rel_host[q] = ||r||/||b|| (-1 for unselected slots). Synchronises. */
+int pdda_mixed_refine(pdda_mixed* m, const unsigned char* sel_host, double* rel_host, void* stream);
+/* Copies the FP64 solution of one slot (3N complex128). */
+int pdda_mixed_slot_result(pdda_mixed* m, int slot, void* P_out_c128, void* stream);
 
 #ifdef __cplusplus
 }

@@ -41,6 +41,10 @@ _SIGS = {
     "pdda_last_error": (_c.c_char_p, []),
     "pdda_fft_length_supported": (_i, [_i]),
     "pdda_green_build": (_i, [_i, _i, _i, _d, _d, _vp, _vp, _vp]),
+    "pdda_green_plan_create": (_i, [_c.POINTER(_vp), _i, _i, _i]),
+    "pdda_green_plan_build": (_i, [_vp, _d, _d, _vp, _vp, _vp]),
+    "pdda_green_plan_bytes": (_ll, [_vp]),
+    "pdda_green_plan_destroy": (_i, [_vp]),
     "pdda_precond_build": (_i, [_i, _i, _i, _d, _d, _vp, _vp, _vp, _vp]),
     "pdda_conv_create": (_i, [_c.POINTER(_vp), _i, _i, _i, _i, _i, _i, _i, _i]),
     "pdda_conv_destroy": (_i, [_vp]),
@@ -62,6 +66,14 @@ _SIGS = {
     "pdda_mixed_bind": (_i, [_vp, _vp, _vp, _ll, _vp, _ll, _vp, _vp, _vp, _ll]),
     "pdda_mixed_solve": (_i, [_vp, _vp, _d, _d, _i, _i, _d, _i, _i, _vp]),
     "pdda_mixed_result": (_i, [_vp, _vp, _c.POINTER(MixedReport), _i, _vp]),
+    "pdda_solver_iterate_graph": (_i, [_vp, _i, _vp]),
+    "pdda_mixed_set_serving": (_i, [_vp, _i]),
+    "pdda_mixed_slot_begin": (_i, [_vp, _i, _vp, _d, _i, _d, _i, _vp]),
+    "pdda_mixed_slot_reload": (_i, [_vp, _i, _d, _i, _d, _i, _vp]),
+    "pdda_mixed_iterate": (_i, [_vp, _i, _i, _vp]),
+    "pdda_mixed_inner_status": (_i, [_vp, _c.POINTER(SlotStatus), _i, _vp]),
+    "pdda_mixed_refine": (_i, [_vp, _c.POINTER(_c.c_ubyte), _c.POINTER(_d), _vp]),
+    "pdda_mixed_slot_result": (_i, [_vp, _i, _vp, _vp]),
     "pdda_mark_shapes": (_i, [_c.POINTER(_d), _i, _i, _i, _i, _d, _i, _vp, _vp]),
     "pdda_trim_bbox": (_i, [_vp, _i, _i, _i, _c.POINTER(_i), _vp]),
     "pdda_crop_pad": (_i, [_vp, _i, _i, _i, _i, _i, _i, _i, _i, _i, _i, _i, _i, _vp, _vp]),

@@ -158,6 +158,36 @@ int pdda_green_build(int nx, int ny, int nz, double k, double d, void* out_c128,
   return guarded([&] { return green_build(nx, ny, nz, k, d, out_c128, out_c64, (hipStream_t)stream); });
 }
 
+struct pdda_green_plan {
+  GreenPlan g;
+};
+
+int pdda_green_plan_create(pdda_green_plan** out, int nx, int ny, int nz) {
+  if (!out) { set_error("green_plan_create: null handle"); return PDDA_ERR_ARG; }
+  *out = nullptr;
+  return guarded([&] {
+    auto* h = new pdda_green_plan();
+    int rc = green_plan_init(&h->g, nx, ny, nz);
+    if (rc) { delete h; return rc; }
+    *out = h;
+    return PDDA_OK;
+  });
+}
+
+int pdda_green_plan_build(pdda_green_plan* g, double k, double d, void* out_c128, void* out_c64, void* stream) {
+  if (!g) { set_error("green_plan_build: null handle"); return PDDA_ERR_ARG; }
+  return guarded([&] { return green_plan_run(&g->g, k, d, out_c128, out_c64, (hipStream_t)stream); });
+}
+
+long long pdda_green_plan_bytes(const pdda_green_plan* g) { return g ? (long long)g->g.bytes : 0; }
+
+int pdda_green_plan_destroy(pdda_green_plan* g) {
+  if (!g) return PDDA_OK;
+  green_plan_free(&g->g);
+  delete g;
+  return PDDA_OK;
+}
+
 int pdda_precond_build(int hx, int hy, int hz, double gre, double gim, const void* g, void* out128, void* out64,
                        void* stream) {
   return guarded([&] { return precond_build(hx, hy, hz, gre, gim, g, out128, out64, (hipStream_t)stream); });
@@ -351,6 +381,64 @@ int pdda_mixed_solve(pdda_mixed* m, const void* E, double tol, double inner_targ
     return m->m.solve(E, tol, inner_target, max_outer, max_inner, ratio, stagnation, check_every,
                       (hipStream_t)stream);
   });
+}
+
+int pdda_solver_iterate_graph(pdda_solver* s, int n_iter, void* stream) {
+  if (!s || n_iter < 0) { set_error("solver_iterate_graph: bad argument"); return PDDA_ERR_ARG; }
+  return guarded([&] {
+    return s->prec ? s->s64.iterate_graph(n_iter, (hipStream_t)stream)
+                   : s->s32.iterate_graph(n_iter, (hipStream_t)stream);
+  });
+}
+
+int pdda_mixed_set_serving(pdda_mixed* m, int on) {
+  if (!m) { set_error("mixed_set_serving: null"); return PDDA_ERR_ARG; }
+  m->m.inner.lockstep = on ? false : true;
+  return PDDA_OK;
+}
+
+int pdda_mixed_slot_begin(pdda_mixed* m, int slot, const void* E64_slot, double inner_target, int max_inner,
+                          double ratio, int stagnation, void* stream) {
+  if (!m || !E64_slot) { set_error("mixed_slot_begin: null argument"); return PDDA_ERR_ARG; }
+  return guarded([&] {
+    return m->m.slot_begin(slot, E64_slot, inner_target, max_inner, ratio, stagnation, (hipStream_t)stream);
+  });
+}
+
+int pdda_mixed_slot_reload(pdda_mixed* m, int slot, double inner_target, int max_inner, double ratio,
+                           int stagnation, void* stream) {
+  if (!m) { set_error("mixed_slot_reload: null"); return PDDA_ERR_ARG; }
+  return guarded([&] {
+    return m->m.slot_reload(slot, inner_target, max_inner, ratio, stagnation, (hipStream_t)stream);
+  });
+}
+
+int pdda_mixed_iterate(pdda_mixed* m, int n_iter, int use_graph, void* stream) {
+  if (!m || n_iter < 0) { set_error("mixed_iterate: bad argument"); return PDDA_ERR_ARG; }
+  return guarded([&] {
+    return use_graph ? m->m.inner.iterate_graph(n_iter, (hipStream_t)stream)
+                     : m->m.inner.iterate(n_iter, (hipStream_t)stream);
+  });
+}
+
+int pdda_mixed_inner_status(pdda_mixed* m, pdda_slot_status* out, int n, void* stream) {
+  if (!m || !out) { set_error("mixed_inner_status: null argument"); return PDDA_ERR_ARG; }
+  int rc = m->m.inner.status((hipStream_t)stream);
+  if (rc) return rc;
+  fill_status(m->m.inner, out, n);
+  return PDDA_OK;
+}
+
+int pdda_mixed_refine(pdda_mixed* m, const unsigned char* sel_host, double* rel_host, void* stream) {
+  if (!m || !sel_host || !rel_host) { set_error("mixed_refine: null argument"); return PDDA_ERR_ARG; }
+  return guarded([&] { return m->m.refine(sel_host, rel_host, (hipStream_t)stream); });
+}
+
+int pdda_mixed_slot_result(pdda_mixed* m, int slot, void* P_out, void* stream) {
+  if (!m || !P_out || slot < 0 || slot >= m->m.nslots) { set_error("mixed_slot_result: bad argument"); return PDDA_ERR_ARG; }
+  PDDA_HIP_CHECK(hipMemcpyAsync(P_out, m->m.x64 + (size_t)slot * m->m.n3, (size_t)m->m.n3 * sizeof(double2),
+                                hipMemcpyDeviceToDevice, (hipStream_t)stream));
+  return PDDA_OK;
 }
 
 int pdda_mixed_result(pdda_mixed* m, void* P, pdda_mixed_report* rep, int n, void* stream) {

@@ -102,6 +102,17 @@ int conv_apply(ConvPlanBase* p, const ConvArgs& a, hipStream_t s);
 // measured with HIP events recorded between the passes on stream s.
 int conv_time_passes(ConvPlanBase* p, const ConvArgs& a, int reps, double* ms5, hipStream_t s);
 int green_build(int nx, int ny, int nz, double k, double d, void* out64, void* out32, hipStream_t s);
+// Reusable workspace for repeated table builds of one box (no allocation or host sync
+// per build; builds of one plan are ordered on one stream).
+struct GreenPlan {
+  int nx = 0, ny = 0, nz = 0;
+  FftPlan px, py, pz;
+  void *twx = nullptr, *twy = nullptr, *twz = nullptr, *G1 = nullptr, *G2 = nullptr;
+  size_t bytes = 0;
+};
+int green_plan_init(GreenPlan* g, int nx, int ny, int nz);
+int green_plan_run(const GreenPlan* g, double k, double d, void* out64, void* out32, hipStream_t s);
+void green_plan_free(GreenPlan* g);
 int precond_build(int hx, int hy, int hz, double gre, double gim, const void* g64, void* out64, void* out32,
                   hipStream_t s);
 

@@ -711,36 +711,42 @@ int conv_time_passes(ConvPlanBase* p, const ConvArgs& a, int reps, double* ms5, 
   return rc;
 }
 
-int green_build(int nx, int ny, int nz, double k, double d, void* out64, void* out32, hipStream_t s) {
+void green_plan_free(GreenPlan* g) {
+  for (void** q : {&g->twx, &g->twy, &g->twz, &g->G1, &g->G2})
+    if (*q) { (void)hipFree(*q); *q = nullptr; }
+}
+
+int green_plan_init(GreenPlan* g, int nx, int ny, int nz) {
   if (nx <= 0 || ny <= 0 || nz <= 0) { set_error("green_build: sizes must be positive"); return PDDA_ERR_ARG; }
-  if (!out64) { set_error("green_build: complex128 output required"); return PDDA_ERR_ARG; }
-  FftPlan px, py, pz;
+  *g = GreenPlan{};
+  g->nx = nx; g->ny = ny; g->nz = nz;
   int rc;
-  if ((rc = make_fft_plan(2 * nx, &px))) return rc;
-  if ((rc = make_fft_plan(2 * ny, &py))) return rc;
-  if ((rc = make_fft_plan(2 * nz, &pz))) return rc;
+  if ((rc = make_fft_plan(2 * nx, &g->px))) return rc;
+  if ((rc = make_fft_plan(2 * ny, &g->py))) return rc;
+  if ((rc = make_fft_plan(2 * nz, &g->pz))) return rc;
   const int cap = tile_cap<double>();
   if (6 * 2 * nz > cap || 2 * nx > cap || 2 * ny > cap) {
     set_error("green_build: FFT length too long for one LDS tile");
     return PDDA_ERR_SIZE;
   }
-  void *twx = nullptr, *twy = nullptr, *twz = nullptr, *G1 = nullptr, *G2 = nullptr;
-  auto cleanup = [&]() {
-    if (twx) (void)hipFree(twx);
-    if (twy) (void)hipFree(twy);
-    if (twz) (void)hipFree(twz);
-    if (G1) (void)hipFree(G1);
-    if (G2) (void)hipFree(G2);
-  };
-  if ((rc = upload_twiddles(2 * nx, 1, &twx)) || (rc = upload_twiddles(2 * ny, 1, &twy)) ||
-      (rc = upload_twiddles(2 * nz, 1, &twz))) { cleanup(); return rc; }
+  if ((rc = upload_twiddles(2 * nx, 1, &g->twx)) || (rc = upload_twiddles(2 * ny, 1, &g->twy)) ||
+      (rc = upload_twiddles(2 * nz, 1, &g->twz))) { green_plan_free(g); return rc; }
   const long long W = 6LL * (nz + 1);
   const size_t g1n = (size_t)(2 * nx) * (2 * ny) * W, g2n = (size_t)(2 * nx) * (ny + 1) * W;
-  if (hipMalloc(&G1, g1n * sizeof(double2)) != hipSuccess || hipMalloc(&G2, g2n * sizeof(double2)) != hipSuccess) {
-    cleanup();
+  if (hipMalloc(&g->G1, g1n * sizeof(double2)) != hipSuccess || hipMalloc(&g->G2, g2n * sizeof(double2)) != hipSuccess) {
+    green_plan_free(g);
     set_error("green_build: out of device memory");
     return PDDA_ERR_OOM;
   }
+  g->bytes = (g1n + g2n) * sizeof(double2);
+  return PDDA_OK;
+}
+
+int green_plan_run(const GreenPlan* g, double k, double d, void* out64, void* out32, hipStream_t s) {
+  if (!out64) { set_error("green_build: complex128 output required"); return PDDA_ERR_ARG; }
+  const int nx = g->nx, ny = g->ny, nz = g->nz;
+  const int cap = tile_cap<double>();
+  const long long W = 6LL * (nz + 1);
   // z pass with generation
   const int Lz = 2 * nz;
   int rows1 = std::max(1, std::min(2 * ny, cap / (6 * Lz)));
@@ -750,8 +756,8 @@ int green_build(int nx, int ny, int nz, double k, double d, void* out64, void* o
   }
   const size_t lds1 = (size_t)(Lz + Lz * 6 * rows1) * sizeof(double2);
   const int nb1 = (2 * nx) * ceil_div(2 * ny, rows1);
-  hipLaunchKernelGGL(k_green_z, dim3(nb1), dim3(FFT_NT), lds1, s, (double2*)G1, nx, ny, nz, k, d, rows1, pz,
-                     (const double2*)twz);
+  hipLaunchKernelGGL(k_green_z, dim3(nb1), dim3(FFT_NT), lds1, s, (double2*)g->G1, nx, ny, nz, k, d, rows1, g->pz,
+                     (const double2*)g->twz);
   // y pass: (2nx, 2ny, W) -> (2nx, ny+1, W)
   {
     AxisArgs ax;
@@ -760,8 +766,9 @@ int green_build(int nx, int ny, int nz, double k, double d, void* out64, void* o
     ax.W = W; ax.ncol = (int)std::min<long long>(W, cap / (2 * ny)); ax.negate = 0;
     const size_t lds = (size_t)(2 * ny + 2 * ny * ax.ncol) * sizeof(double2);
     const long long nblk = (long long)ax.n_outer * ceil_div(ax.W, ax.ncol);
-    hipLaunchKernelGGL((k_axis<double, false, true>), dim3((unsigned)nblk), dim3(FFT_NT), lds, s, (const double2*)G1,
-                       (double2*)G2, ax, py, (const double2*)twy, (const SlotCtl*)nullptr, (int)NEED_ALL);
+    hipLaunchKernelGGL((k_axis<double, false, true>), dim3((unsigned)nblk), dim3(FFT_NT), lds, s,
+                       (const double2*)g->G1, (double2*)g->G2, ax, g->py, (const double2*)g->twy,
+                       (const SlotCtl*)nullptr, (int)NEED_ALL);
   }
   // x pass: (2nx, (ny+1)W) -> (nx+1, (ny+1)W)
   {
@@ -772,19 +779,30 @@ int green_build(int nx, int ny, int nz, double k, double d, void* out64, void* o
     ax.negate = 0;
     const size_t lds = (size_t)(2 * nx + 2 * nx * ax.ncol) * sizeof(double2);
     const long long nblk = ceil_div(ax.W, ax.ncol);
-    hipLaunchKernelGGL((k_axis<double, false, true>), dim3((unsigned)nblk), dim3(FFT_NT), lds, s, (const double2*)G2,
-                       (double2*)out64, ax, px, (const double2*)twx, (const SlotCtl*)nullptr, (int)NEED_ALL);
+    hipLaunchKernelGGL((k_axis<double, false, true>), dim3((unsigned)nblk), dim3(FFT_NT), lds, s,
+                       (const double2*)g->G2, (double2*)out64, ax, g->px, (const double2*)g->twx,
+                       (const SlotCtl*)nullptr, (int)NEED_ALL);
   }
   if (out32) {
     const long long n = (long long)(nx + 1) * (ny + 1) * W;
     hipLaunchKernelGGL(k_cast64to32, dim3(1024), dim3(256), 0, s, (const double2*)out64, (float2*)out32, n);
   }
-  hipError_t e = hipGetLastError();
-  // the temporaries must outlive the kernels that use them
+  PDDA_LAUNCH_CHECK();
+  return PDDA_OK;
+}
+
+int green_build(int nx, int ny, int nz, double k, double d, void* out64, void* out32, hipStream_t s) {
+  if (!out64) { set_error("green_build: complex128 output required"); return PDDA_ERR_ARG; }
+  GreenPlan g;
+  int rc = green_plan_init(&g, nx, ny, nz);
+  if (rc) return rc;
+  rc = green_plan_run(&g, k, d, out64, out32, s);
+  // the workspace must outlive the kernels that use it
   hipError_t e2 = hipStreamSynchronize(s);
-  cleanup();
-  if (e != hipSuccess || e2 != hipSuccess) {
-    set_error(std::string("green_build: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+  green_plan_free(&g);
+  if (rc) return rc;
+  if (e2 != hipSuccess) {
+    set_error(std::string("green_build: ") + hipGetErrorString(e2));
     return PDDA_ERR_HIP;
   }
   return PDDA_OK;

@@ -40,6 +40,11 @@ template <typename T> struct SolverT {
   int sidx = 0;
   long long host_iter = 0;
   bool lockstep = true;  // all slots started together: replacement only at iteration 1
+  hipGraphExec_t graph_exec = nullptr;
+  int graph_n = 0;
+  hipStream_t graph_stream = nullptr;
+  bool graph_replace = false;
+  int graph_sidx = 0;
   // bound operands (device pointers, plan precision)
   const C<T>* ghat = nullptr;
   long long ghat_stride = 0;
@@ -53,6 +58,9 @@ template <typename T> struct SolverT {
   void release();
   int load(int slot, const void* E, double target, int max_iter, double ratio, int stagnation, hipStream_t s);
   int iterate(int n_iter, hipStream_t s);
+  // Same iterations replayed from a captured hipGraph (n_iter even; captured on first
+  // use and whenever n_iter or the stream changes).
+  int iterate_graph(int n_iter, hipStream_t s);
   int status(hipStream_t s);
   int result(void* out, hipStream_t s);
   int conv(ConvPlanBase* plan, int mode, const C<T>* x, C<T>* out, const C<T>* table, long long tstride, int dots,
@@ -83,7 +91,16 @@ struct MixedSolver {
 
   int init(int nx, int ny, int nz, int px, int py, int pz, int nslots, int precon);
   void release();
+  uint8_t* sel_dev = nullptr;
+  uint8_t* sel_pinned = nullptr;
   int residual64(hipStream_t s);
+  // Serving: independent per-slot lifecycles (the inner solver runs with lockstep off).
+  int slot_begin(int q, const void* E64_slot, double inner_target, int max_inner, double ratio, int stagnation,
+                 hipStream_t s);
+  int slot_reload(int q, double inner_target, int max_inner, double ratio, int stagnation, hipStream_t s);
+  // FP64 refinement of the selected slots: x += ||r|| * (inner result), r = b - A x;
+  // writes ||r||/||b|| per selected slot (-1 for the others). Synchronises the stream.
+  int refine(const uint8_t* sel_host, double* rel_host, hipStream_t s);
   int solve(const void* E64, double tol, double inner_target, int max_outer, int max_inner, double ratio,
             int stagnation, int check_every, hipStream_t s);
 };

@@ -15,7 +15,7 @@ constexpr int VNT = 256;  // threads per workgroup of the vector kernels
 // ---------------------------------------------------------------------------------
 // Reductions for vector kernels (same ticket scheme as the convolution epilogue).
 // ---------------------------------------------------------------------------------
-enum : int { FIN_STORE = 0, FIN_INIT = 1, FIN_NORM = 2, FIN_CONTROL = 3 };
+enum : int { FIN_STORE = 0, FIN_INIT = 1, FIN_NORM = 2, FIN_CONTROL = 3, FIN_NORM_INIT = 4 };
 
 __device__ inline void finalize_vec(SlotCtl& c, int kind, const double* re, const double* im, double p0,
                                     double p1, int ip0, int ip1) {
@@ -53,9 +53,13 @@ __device__ inline void finalize_vec(SlotCtl& c, int kind, const double* re, cons
       c.flags = F_ACTIVE;
       c.status = ST_RUNNING;
     }
-  } else {  // FIN_NORM: refinement residual norm
+  } else {  // FIN_NORM / FIN_NORM_INIT: refinement residual norm (and ||b||)
     c.sums[0] = cd{re[0], 0.0};
     c.current_norm = sqrt(re[0]);
+    if (kind == FIN_NORM_INIT) {
+      c.initial_norm = c.current_norm;
+      c.flags = c.current_norm > 0.0 ? F_ACTIVE : 0;
+    }
   }
 }
 
@@ -221,11 +225,13 @@ __global__ __launch_bounds__(VNT) void k_upd3(VecSet<T> v, long long n3, const S
   }
 }
 
-// out = (status == converged) ? x : best
+// out = (status == converged) ? x : best, for the slots selected by sel (all if null)
 template <typename T>
 __global__ __launch_bounds__(VNT) void k_select(const C<T>* __restrict__ x, const C<T>* __restrict__ best,
-                                                C<T>* __restrict__ out, long long n3, const SlotCtl* ctl, int nwg) {
+                                                C<T>* __restrict__ out, long long n3, const SlotCtl* ctl, int nwg,
+                                                const uint8_t* __restrict__ sel) {
   const int slot = blockIdx.y;
+  if (sel && !sel[slot]) return;
   const bool conv = ctl[slot].status == ST_CONVERGED;
   const C<T>* src = conv ? x : best;
   const long long off = (long long)slot * n3;
@@ -238,8 +244,8 @@ __global__ __launch_bounds__(VNT) void k_select(const C<T>* __restrict__ x, cons
 // ---------------------------------------------------------------------------------
 // E32 = r64 / ||r64|| for slots still refining; stores the scale in ctl.rhs_scale.
 __global__ __launch_bounds__(VNT) void k_refine_rhs(const double2* __restrict__ r, float2* __restrict__ e,
-                                                    long long n3, SlotCtl* ctl64, int nwg) {
-  const int slot = blockIdx.y;
+                                                    long long n3, SlotCtl* ctl64, int nwg, int slot0) {
+  const int slot = slot0 + blockIdx.y;
   const double nr = ctl64[slot].current_norm;
   const double sc = nr > 0.0 ? 1.0 / nr : 0.0;
   const long long off = (long long)slot * n3;
@@ -269,8 +275,8 @@ __global__ __launch_bounds__(VNT) void k_refine_acc(double2* __restrict__ x, con
 __global__ __launch_bounds__(VNT) void k_refine_init(const double2* __restrict__ E, const uint8_t* __restrict__ mask,
                                                      long long site_stride, double2* __restrict__ b,
                                                      double2* __restrict__ x, double2* __restrict__ r, long long n3,
-                                                     cd* partial, int* counters, SlotCtl* ctl64, int nwg) {
-  const int slot = blockIdx.y;
+                                                     cd* partial, int* counters, SlotCtl* ctl64, int nwg, int slot0) {
+  const int slot = slot0 + blockIdx.y;
   const long long off = (long long)slot * n3;
   const uint8_t* m = mask + (long long)slot * site_stride;
   double re[1] = {0.0}, im[1] = {0.0};
@@ -284,6 +290,12 @@ __global__ __launch_bounds__(VNT) void k_refine_init(const double2* __restrict__
     re[0] += bv.x * bv.x + bv.y * bv.y;
   }
   vec_partial<1>(re, im, partial + (size_t)slot * nwg * 6, blockIdx.x);
+}
+
+// ctl64 flags = F_ACTIVE for the selected slots, 0 otherwise.
+__global__ void k_select_flags(SlotCtl* ctl64, const uint8_t* __restrict__ sel, int nslots) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < nslots) ctl64[q].flags = sel[q] ? F_ACTIVE : 0;
 }
 
 // ---------------------------------------------------------------------------------
@@ -320,6 +332,8 @@ int SolverT<T>::init(int nx_, int ny_, int nz_, int px_, int py_, int pz_, int n
 
 template <typename T>
 void SolverT<T>::release() {
+  if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+  graph_exec = nullptr;
   conv_plan_free(&mv);
   if (precon) conv_plan_free(&pc);
   C<T>* bufs[] = {vs.b, vs.r, vs.rhat, vs.p, vs.x, vs.best, vs.w, vs.u, vs.z, vs.s0, vs.s1,
@@ -342,8 +356,12 @@ int SolverT<T>::load(int slot, const void* E, double target, int max_iter, doubl
                      partial, counters, ctl, target, ratio, max_iter, stagnation, nwg);
   hipLaunchKernelGGL(k_vec_finalize<1>, dim3(ns), dim3(VNT), 0, s, (const cd*)partial, nwg, ctl, s0, (int)FIN_INIT, 0,
                      target, ratio, max_iter, stagnation);
-  sidx = 0;
-  host_iter = 0;
+  // A single-slot load while other slots run (serving) keeps the ping-pong parity: the
+  // loaded slot's s0 and s1 are both zero, so either can be its s_prev.
+  if (lockstep || slot < 0) {
+    sidx = 0;
+    host_iter = 0;
+  }
   PDDA_LAUNCH_CHECK();
   return PDDA_OK;
 }
@@ -431,6 +449,36 @@ int SolverT<T>::iterate(int n_iter, hipStream_t s) {
 }
 
 template <typename T>
+int SolverT<T>::iterate_graph(int n_iter, hipStream_t s) {
+  if (n_iter <= 0) return PDDA_OK;
+  if (n_iter % 2) { set_error("iterate_graph: n_iter must be even (s/s_prev ping-pong)"); return PDDA_ERR_ARG; }
+  if (lockstep && host_iter < 2) return iterate(n_iter, s);  // replacement step runs eagerly
+  if (!graph_exec || graph_n != n_iter || graph_stream != s || graph_replace != (!lockstep) || graph_sidx != sidx) {
+    if (graph_exec) { (void)hipGraphExecDestroy(graph_exec); graph_exec = nullptr; }
+    hipGraph_t gr = nullptr;
+    const long long hi = host_iter;
+    const int si = sidx;
+    PDDA_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+    int rc = iterate(n_iter, s);
+    hipError_t e = hipStreamEndCapture(s, &gr);
+    if (rc) { if (gr) (void)hipGraphDestroy(gr); return rc; }
+    if (e != hipSuccess) { set_error(std::string("graph capture: ") + hipGetErrorString(e)); return PDDA_ERR_HIP; }
+    e = hipGraphInstantiate(&graph_exec, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    if (e != hipSuccess) { set_error(std::string("graph instantiate: ") + hipGetErrorString(e)); return PDDA_ERR_HIP; }
+    graph_n = n_iter;
+    graph_stream = s;
+    graph_replace = !lockstep;
+    graph_sidx = si;
+    host_iter = hi;  // capture enqueued nothing
+    sidx = si;
+  }
+  PDDA_HIP_CHECK(hipGraphLaunch(graph_exec, s));
+  host_iter += n_iter;
+  return PDDA_OK;
+}
+
+template <typename T>
 int SolverT<T>::status(hipStream_t s) {
   PDDA_HIP_CHECK(hipMemcpyAsync(ctl_host, ctl, nslots * sizeof(SlotCtl), hipMemcpyDeviceToHost, s));
   PDDA_HIP_CHECK(hipStreamSynchronize(s));
@@ -440,7 +488,8 @@ int SolverT<T>::status(hipStream_t s) {
 template <typename T>
 int SolverT<T>::result(void* out, hipStream_t s) {
   const dim3 vg(nwg, nslots);
-  hipLaunchKernelGGL(k_select<T>, vg, dim3(VNT), 0, s, vs.x, vs.best, vs.tmp, n3, ctl, nwg);
+  hipLaunchKernelGGL(k_select<T>, vg, dim3(VNT), 0, s, vs.x, vs.best, vs.tmp, n3, ctl, nwg,
+                     (const uint8_t*)nullptr);
   if (precon) {
     int rc = conv(&pc, MODE_PRECOND, vs.tmp, (C<T>*)out, mhat, mhat_stride, DOT_NONE, nullptr, nullptr, NEED_ALL, s);
     if (rc) return rc;
@@ -480,6 +529,10 @@ int MixedSolver::init(int nx, int ny, int nz, int px, int py, int pz, int nslots
 }
 
 void MixedSolver::release() {
+  if (sel_dev) (void)hipFree(sel_dev);
+  if (sel_pinned) (void)hipHostFree(sel_pinned);
+  sel_dev = nullptr;
+  sel_pinned = nullptr;
   inner.release();
   conv_plan_free(&mv64);
   void* bufs[] = {b64, x64, r64, e32, d32, ctl64, partial, counters};
@@ -515,7 +568,7 @@ int MixedSolver::solve(const void* E64, double tol, double inner_target, int max
   int rc;
   const dim3 vg(nwg, nslots);
   hipLaunchKernelGGL(k_refine_init, vg, dim3(VNT), 0, s, (const double2*)E64, inner.mask, inner.site_stride, b64,
-                     x64, r64, n3, partial, counters, ctl64, nwg);
+                     x64, r64, n3, partial, counters, ctl64, nwg, 0);
   hipLaunchKernelGGL(k_vec_finalize<1>, dim3(nslots), dim3(VNT), 0, s, (const cd*)partial, nwg, ctl64, 0,
                      (int)FIN_NORM, 0, 0.0, 0.0, 0, 0);
   PDDA_LAUNCH_CHECK();
@@ -538,7 +591,7 @@ int MixedSolver::solve(const void* E64, double tol, double inner_target, int max
     for (int q = 0; q < nslots; ++q) any |= !done[q];
     if (!any) break;
     // inner rhs = r / ||r|| for refining slots; other slots get max_iter 0 (idle)
-    hipLaunchKernelGGL(k_refine_rhs, vg, dim3(VNT), 0, s, r64, e32, n3, ctl64, nwg);
+    hipLaunchKernelGGL(k_refine_rhs, vg, dim3(VNT), 0, s, r64, e32, n3, ctl64, nwg, 0);
     PDDA_LAUNCH_CHECK();
     for (int q = 0; q < nslots; ++q) {
       if ((rc = inner.load(q, e32 + (size_t)q * n3, inner_target, done[q] ? 0 : max_inner, ratio, stagnation, s)))
@@ -577,6 +630,62 @@ int MixedSolver::solve(const void* E64, double tol, double inner_target, int max
     PDDA_HIP_CHECK(hipMemcpyAsync(ctl64, ctl64_host, nslots * sizeof(SlotCtl), hipMemcpyHostToDevice, s));
   }
   PDDA_HIP_CHECK(hipStreamSynchronize(s));
+  return PDDA_OK;
+}
+
+int MixedSolver::slot_begin(int q, const void* E64_slot, double inner_target, int max_inner, double ratio,
+                            int stagnation, hipStream_t s) {
+  if (q < 0 || q >= nslots) { set_error("slot_begin: slot out of range"); return PDDA_ERR_ARG; }
+  if (!inner.mask) { set_error("slot_begin: operands not bound"); return PDDA_ERR_STATE; }
+  // k_refine_init indexes E at slot * n3: shift the base so that the slot reads E64_slot
+  const double2* Eb = (const double2*)E64_slot - (size_t)q * n3;
+  hipLaunchKernelGGL(k_refine_init, dim3(nwg, 1), dim3(VNT), 0, s, Eb, inner.mask, inner.site_stride, b64, x64, r64,
+                     n3, partial, counters, ctl64, nwg, q);
+  hipLaunchKernelGGL(k_vec_finalize<1>, dim3(1), dim3(VNT), 0, s, (const cd*)partial, nwg, ctl64, q,
+                     (int)FIN_NORM_INIT, 0, 0.0, 0.0, 0, 0);
+  PDDA_LAUNCH_CHECK();
+  return slot_reload(q, inner_target, max_inner, ratio, stagnation, s);
+}
+
+int MixedSolver::slot_reload(int q, double inner_target, int max_inner, double ratio, int stagnation,
+                             hipStream_t s) {
+  if (q < 0 || q >= nslots) { set_error("slot_reload: slot out of range"); return PDDA_ERR_ARG; }
+  hipLaunchKernelGGL(k_refine_rhs, dim3(nwg, 1), dim3(VNT), 0, s, r64, e32, n3, ctl64, nwg, q);
+  PDDA_LAUNCH_CHECK();
+  return inner.load(q, e32 + (size_t)q * n3, inner_target, max_inner, ratio, stagnation, s);
+}
+
+int MixedSolver::refine(const uint8_t* sel_host, double* rel_host, hipStream_t s) {
+  if (!sel_dev) {
+    PDDA_HIP_CHECK(hipMalloc(&sel_dev, nslots));
+    PDDA_HIP_CHECK(hipHostMalloc((void**)&sel_pinned, nslots, hipHostMallocDefault));
+  }
+  PDDA_HIP_CHECK(hipStreamSynchronize(s));  // sel_pinned may still feed an earlier copy
+  std::memcpy(sel_pinned, sel_host, nslots);
+  PDDA_HIP_CHECK(hipMemcpyAsync(sel_dev, sel_pinned, nslots, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_select_flags, dim3((nslots + 63) / 64), dim3(64), 0, s, ctl64, (const uint8_t*)sel_dev, nslots);
+  const dim3 vg(nwg, nslots);
+  // d32 = selected inner results (M y when preconditioned)
+  if (inner.precon) {
+    hipLaunchKernelGGL(k_select<float>, vg, dim3(VNT), 0, s, inner.vs.x, inner.vs.best, inner.vs.tmp, n3, inner.ctl,
+                       nwg, (const uint8_t*)sel_dev);
+    int rc = inner.conv(&inner.pc, MODE_PRECOND, inner.vs.tmp, d32, inner.mhat, inner.mhat_stride, DOT_NONE, nullptr,
+                        nullptr, NEED_ALL, s);
+    if (rc) return rc;
+  } else {
+    hipLaunchKernelGGL(k_select<float>, vg, dim3(VNT), 0, s, inner.vs.x, inner.vs.best, d32, n3, inner.ctl, nwg,
+                       (const uint8_t*)sel_dev);
+  }
+  hipLaunchKernelGGL(k_refine_acc, vg, dim3(VNT), 0, s, x64, d32, n3, ctl64, nwg);
+  PDDA_LAUNCH_CHECK();
+  int rc = residual64(s);
+  if (rc) return rc;
+  PDDA_HIP_CHECK(hipMemcpyAsync(ctl64_host, ctl64, nslots * sizeof(SlotCtl), hipMemcpyDeviceToHost, s));
+  PDDA_HIP_CHECK(hipStreamSynchronize(s));
+  for (int q = 0; q < nslots; ++q) {
+    const SlotCtl& c = ctl64_host[q];
+    rel_host[q] = (sel_host[q] && c.initial_norm > 0.0) ? c.current_norm / c.initial_norm : -1.0;
+  }
   return PDDA_OK;
 }
 

@@ -1,0 +1,386 @@
+"""Continuous-batching solve service: DDA solves of uneven targets as they arrive.
+
+Requests are binned by padded box: each axis is rounded up to a size whose doubled FFT
+length has a compile-time plan. Empty padding sites carry inv_alpha = 0 and mask = 0,
+so the solution on the target is unchanged. Each bin is one mixed-precision solver
+(complex64 GPBiCG + FP64 refinement) whose slots have independent lifecycles and
+their own interaction tables, polarisabilities, masks and incident fields:
+
+    admit -> complex64 inner iterations (captured hipGraph, all running slots)
+          -> when a slot's inner solve stops: FP64 refinement of that slot
+          -> FP64 residual < tol: result, slot freed; else inner solve reloaded
+
+Bins run on their own streams, so work of different bins overlaps on the device.
+A new bin is created only if its memory fits the HBM budget; otherwise the request
+waits for a slot. Latency is measured from arrival to result-ready.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import time
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Tuple
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import targets as T
+from ._lib import check, lib, stream_ptr
+from .operators import C64, C128
+from .solver import DEFAULT_STAGNATION
+from .workloads import a_eff
+
+PADDED_SIZES = (8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256)
+
+
+def padded_size(n: int) -> int:
+    for s in PADDED_SIZES:
+        if s >= n:
+            return s
+    raise ValueError(f"box dimension {n} exceeds the largest served size {PADDED_SIZES[-1]}")
+
+
+@dataclass
+class Request:
+    rid: int
+    shape: dict           # {"type": sphere|ellipsoid|rectangle, ...geometry...}
+    m: complex
+    x: float              # size parameter k * a_eff
+    kdir: np.ndarray
+    epol: np.ndarray
+    arrival: float = 0.0  # seconds after the start of the run
+
+
+@dataclass
+class Result:
+    rid: int
+    P: torch.Tensor       # (nx, ny, nz, 3) complex128 on the trimmed box
+    grid: tuple
+    k: float
+    rel_residual: float
+    outer: int
+    inner_iters: int
+    arrival: float
+    done: float
+    ok: bool
+    admitted: float = 0.0  # when the request got a slot (seconds after the start)
+    box: tuple = ()        # padded box of its bin
+
+    @property
+    def latency(self) -> float:
+        return self.done - self.arrival
+
+
+def _unit(v):
+    v = np.asarray(v, dtype=np.float64)
+    return v / np.linalg.norm(v)
+
+
+def make_requests(n: int = 200, seed: int = 0, rate: Optional[float] = None,
+                  edges=(24, 32, 48, 64, 96)) -> List[Request]:
+    """The serving mix: spheres / spheroids / cuboids, long edge L in {24,32,48,64,96},
+    short edges L/ar rounded to even >= 2 (ar ~ U[1,3]; spheres use L), Re m ~ U[1.2,2.0],
+    Im m log-uniform in [1e-3, 0.1], x ~ U[1, min(15, 0.5 a_eff/|m|)] (|m| kd <= 0.5,
+    L resampled when that bound is below 1), random direction and polarisation angle.
+    Arrivals: Poisson with `rate` per second (all at t = 0 when rate is None)."""
+    rng = np.random.default_rng(seed)
+    reqs: List[Request] = []
+    t = 0.0
+    kinds = ["sphere", "ellipsoid", "rectangle"]
+    for rid in range(n):
+        kind = kinds[int(rng.integers(0, 3))]
+        while True:
+            L = int(rng.choice(list(edges)))
+            ar = float(rng.uniform(1.0, 3.0))
+            S = max(2, int(2 * round(L / ar / 2.0)))
+            if kind == "sphere":
+                shape = {"type": "sphere", "center": [L / 2] * 3, "radius": L / 2}
+                vol = 4.0 / 3.0 * math.pi * (L / 2) ** 3
+            elif kind == "ellipsoid":
+                shape = {"type": "ellipsoid", "center": [S / 2, S / 2, L / 2], "semi_axes": [S / 2, S / 2, L / 2]}
+                vol = 4.0 / 3.0 * math.pi * (S / 2) ** 2 * (L / 2)
+            else:
+                shape = {"type": "rectangle", "center": [S / 2, S / 2, L / 2], "dimensions": [S, S, L]}
+                vol = float(S * S * L)
+            m = complex(rng.uniform(1.2, 2.0), 10 ** rng.uniform(-3, -1))
+            ae = (3.0 * vol / (4.0 * math.pi)) ** (1.0 / 3.0)
+            xmax = min(15.0, 0.5 * ae / abs(m))
+            if xmax >= 1.0:
+                break
+        x = float(rng.uniform(1.0, xmax))
+        v = rng.standard_normal(3)
+        kd = _unit(v)
+        e1 = np.cross([0.0, 0.0, 1.0], kd)
+        if np.linalg.norm(e1) < 1e-6:
+            e1 = np.cross([1.0, 0.0, 0.0], kd)
+        e1 = _unit(e1)
+        e2 = np.cross(kd, e1)
+        psi = float(rng.uniform(0.0, 2.0 * math.pi))
+        ep = math.cos(psi) * e1 + math.sin(psi) * e2
+        if rate is not None:
+            t += float(rng.exponential(1.0 / rate))
+        reqs.append(Request(rid, shape, m, x, kd, ep, t if rate is not None else 0.0))
+    return reqs
+
+
+def prepare_target(req: Request):
+    """Occupancy of the request's target on its trimmed box (device), dipole count and k
+    from x = k a_eff; alpha = LDR polarisability (direction independent) on the sites."""
+    shp = dict(req.shape)
+    typ = shp.pop("type")
+    sm = T.ShapeManager()
+    sm.add_shape(typ, n=2.0, **shp)
+    for s in sm.shapes.values():
+        s.value = 1.0
+        s.center = list(np.ceil(np.asarray(s.center, dtype=np.float64)))
+    grid = sm.calculate_grid_extents(1.0)
+    occ = T.trim_zero_faces(T.mark_shapes(grid, 1.0, sm.shapes.values(), True, "value"))
+    nd = int((occ != 0).sum().item())
+    # x = k a_eff with a_eff from the dipole count; |m| k d <= 0.5 holds on that count
+    k = min(req.x / a_eff(nd), 0.5 / abs(req.m))
+    alpha = T.calculate_alpha(req.m, 1.0, k, req.kdir, req.epol)
+    return occ, nd, k, complex(alpha)
+
+
+def slot_bytes(box) -> int:
+    """HBM held per slot of a bin (vectors, FFT workspaces, tables, site arrays)."""
+    nx, ny, nz = box
+    N = nx * ny * nz
+    Np = (nx + 1) * (ny + 1) * (nz + 1)
+    c64, c128 = 8, 16
+    inner = 17 * 3 * N * c64 + 18 * N * c64           # Krylov vectors + c64 FFT workspace
+    refine = 3 * 3 * N * c128 + 2 * 3 * N * c64 + 18 * N * c128  # b, x, r, e, d + c128 workspace
+    operands = 6 * Np * (c64 + c128) + N * (c64 + c128 + 1) + 3 * N * c128
+    return inner + refine + operands
+
+
+class Bin:
+    """One padded box size: a mixed solver with `cap` independent slots."""
+
+    def __init__(self, box: Tuple[int, int, int], cap: int, tol: float, inner_target: float, max_inner: int,
+                 max_outer: int, check_every: int, use_graph: bool):
+        self.box = tuple(box)
+        self.cap = cap
+        self.tol, self.inner_target, self.max_inner = tol, inner_target, max_inner
+        self.max_outer, self.check_every, self.use_graph = max_outer, check_every, use_graph
+        nx, ny, nz = self.box
+        self.N = nx * ny * nz
+        self.tab = (nx + 1) * (ny + 1) * (nz + 1) * 6
+        dev = torch.device("cuda", torch.cuda.current_device())
+        self.stream = torch.cuda.Stream(device=dev)  # default priority
+        with torch.cuda.stream(self.stream):
+            self.g64 = torch.zeros((cap, self.tab), dtype=C128, device=dev)
+            self.g32 = torch.zeros((cap, self.tab), dtype=C64, device=dev)
+            self.ia64 = torch.zeros((cap, self.N), dtype=C128, device=dev)
+            self.ia32 = torch.zeros((cap, self.N), dtype=C64, device=dev)
+            self.mask = torch.zeros((cap, self.N), dtype=torch.uint8, device=dev)
+            self.E = torch.zeros((cap, 3 * self.N), dtype=C128, device=dev)
+        h = ctypes.c_void_p()
+        check(lib().pdda_mixed_create(ctypes.byref(h), nx, ny, nz, 0, 0, 0, cap, 0))
+        self._h = h
+        check(lib().pdda_mixed_set_serving(h, 1))
+        vp = lambda t: ctypes.c_void_p(t.data_ptr())
+        check(lib().pdda_mixed_bind(h, vp(self.g64), vp(self.g32), self.tab, None, 0, vp(self.ia64), vp(self.ia32),
+                                    vp(self.mask), self.N))
+        gp = ctypes.c_void_p()
+        check(lib().pdda_green_plan_create(ctypes.byref(gp), nx, ny, nz))
+        self._gp = gp
+        self.slots: List[Optional[dict]] = [None] * cap
+        self.bytes = cap * slot_bytes(self.box) + int(lib().pdda_green_plan_bytes(gp))
+        self.event = torch.cuda.Event()
+        self.inflight = False  # iterations launched and not yet collected
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib._LIB is not None:
+            _lib._LIB.pdda_mixed_destroy(h)
+            self._h = None
+        gp = getattr(self, "_gp", None)
+        if gp is not None and gp.value and _lib._LIB is not None:
+            _lib._LIB.pdda_green_plan_destroy(gp)
+            self._gp = None
+
+    def free_slot(self) -> Optional[int]:
+        for q, s in enumerate(self.slots):
+            if s is None:
+                return q
+        return None
+
+    def busy(self) -> bool:
+        return any(s is not None for s in self.slots)
+
+    def admit(self, req: Request, prep, q: int, now: float = 0.0):
+        occ, nd, k, alpha = prep
+        bx = tuple(occ.shape)
+        nx, ny, nz = self.box
+        sp = stream_ptr(self.stream)
+        self.stream.wait_stream(torch.cuda.current_stream())
+        occ.record_stream(self.stream)
+        with torch.cuda.stream(self.stream):
+            a = torch.zeros(self.box, dtype=C128, device=occ.device)
+            a[:bx[0], :bx[1], :bx[2]] = (occ != 0).to(C128) * alpha
+            nzm = a != 0
+            ia = torch.where(nzm, 1.0 / torch.where(nzm, a, torch.ones_like(a)), torch.zeros_like(a)).reshape(-1)
+            self.ia64[q].copy_(ia)
+            self.ia32[q].copy_(ia.to(C64))
+            self.mask[q].copy_(nzm.reshape(-1).to(torch.uint8))
+            E = T.plane_waves(self.box, k, req.kdir, req.epol)[0]
+            self.E[q].copy_(E.reshape(-1))
+        check(lib().pdda_green_plan_build(self._gp, float(k), 1.0, ctypes.c_void_p(self.g64[q].data_ptr()),
+                                          ctypes.c_void_p(self.g32[q].data_ptr()), sp))
+        check(lib().pdda_mixed_slot_begin(self._h, q, ctypes.c_void_p(self.E[q].data_ptr()), self.inner_target,
+                                          self.max_inner, 100.0, DEFAULT_STAGNATION, sp))
+        self.slots[q] = {"req": req, "grid": bx, "k": k, "outer": 0, "inner": 0, "admitted": now}
+
+    def advance(self):
+        """Launches check_every inner iterations of every running slot; non-blocking."""
+        if self.busy():
+            check(lib().pdda_mixed_iterate(self._h, self.check_every, 1 if self.use_graph else 0,
+                                           stream_ptr(self.stream)))
+            self.event.record(self.stream)
+            self.inflight = True
+
+    def ready(self) -> bool:
+        """True when the launched iterations have finished on the device."""
+        return not self.inflight or self.event.query()
+
+    def collect(self, t0: float) -> List[Result]:
+        """Polls the slot states; refines slots whose inner solve stopped; returns results."""
+        if not self.busy():
+            return []
+        st = (_lib.SlotStatus * self.cap)()
+        check(lib().pdda_mixed_inner_status(self._h, st, self.cap, stream_ptr(self.stream)))
+        sel = (ctypes.c_ubyte * self.cap)()
+        any_sel = False
+        for q, s in enumerate(self.slots):
+            if s is not None and st[q].status != _lib.ST_RUNNING:
+                sel[q] = 1
+                any_sel = True
+                s["inner"] += abs(st[q].iters)
+        if not any_sel:
+            return []
+        rel = (ctypes.c_double * self.cap)()
+        check(lib().pdda_mixed_refine(self._h, sel, rel, stream_ptr(self.stream)))
+        out = []
+        for q in range(self.cap):
+            if not sel[q]:
+                continue
+            s = self.slots[q]
+            s["outer"] += 1
+            r = rel[q]
+            done = r < self.tol or not math.isfinite(r) or s["outer"] >= self.max_outer
+            if not done:
+                check(lib().pdda_mixed_slot_reload(self._h, q, self.inner_target, self.max_inner, 100.0,
+                                                   DEFAULT_STAGNATION, stream_ptr(self.stream)))
+                continue
+            P = torch.empty(3 * self.N, dtype=C128, device=self.E.device)
+            check(lib().pdda_mixed_slot_result(self._h, q, ctypes.c_void_p(P.data_ptr()), stream_ptr(self.stream)))
+            self.stream.synchronize()
+            gx, gy, gz = s["grid"]
+            Pbox = P.reshape(*self.box, 3)[:gx, :gy, :gz].contiguous()
+            req = s["req"]
+            out.append(Result(req.rid, Pbox, s["grid"], s["k"], r, s["outer"], s["inner"], req.arrival,
+                              time.perf_counter() - t0, math.isfinite(r) and r < self.tol, s["admitted"],
+                              self.box))
+            self.slots[q] = None
+        return out
+
+
+class ServingEngine:
+    """Admits requests as they arrive (wall clock), keeps every bin's slots iterating."""
+
+    def __init__(self, hbm_budget: float = 64e9, slots_per_bin: int = 8, tol: float = 1e-6,
+                 inner_target: float = 1e-3, max_inner: int = 3000, max_outer: int = 12, check_every: int = 8,
+                 use_graph: bool = True):
+        _lib.require_gpu()
+        if check_every % 2:
+            raise ValueError("check_every must be even")
+        self.budget = hbm_budget
+        self.slots_per_bin = slots_per_bin
+        self.kw = dict(tol=tol, inner_target=inner_target, max_inner=max_inner, max_outer=max_outer,
+                       check_every=check_every, use_graph=use_graph)
+        self.bins: Dict[Tuple[int, int, int], Bin] = {}
+        self.bin_create_s = 0.0  # host time spent creating bins (allocation, tile autotuning)
+
+    def used_bytes(self) -> int:
+        return sum(b.bytes for b in self.bins.values())
+
+    def _bin_for(self, box) -> Optional[Bin]:
+        b = self.bins.get(box)
+        if b is not None:
+            return b
+        need = self.slots_per_bin * slot_bytes(box)
+        if self.used_bytes() + need > self.budget:
+            # release idle bins of other sizes to make room
+            for key in [k for k, v in self.bins.items() if not v.busy()]:
+                del self.bins[key]
+                if self.used_bytes() + need <= self.budget:
+                    break
+        if self.used_bytes() + need > self.budget:
+            cap = int((self.budget - self.used_bytes()) // slot_bytes(box))
+            if cap < 1:
+                return None
+        else:
+            cap = self.slots_per_bin
+        t = time.perf_counter()
+        b = Bin(box, cap, **self.kw)
+        self.bin_create_s += time.perf_counter() - t
+        self.bins[box] = b
+        return b
+
+    def run(self, requests: List[Request], log=None) -> Tuple[List[Result], float]:
+        """Serves the requests (arrival times relative to the start); returns results and
+        the wall time of the run. Each bin is polled without blocking: a bin whose launched
+        iterations have finished is collected (refinement, results, reloads) and relaunched
+        at once, so bins of different sizes progress at their own pace."""
+        pending = sorted(requests, key=lambda r: r.arrival)
+        preps = {}
+        results: List[Result] = []
+        waiting: List[Request] = []
+        t0 = time.perf_counter()
+        pi = 0
+        while pi < len(pending) or waiting or any(b.busy() for b in self.bins.values()):
+            now = time.perf_counter() - t0
+            while pi < len(pending) and pending[pi].arrival <= now:
+                waiting.append(pending[pi])
+                pi += 1
+            still = []
+            for req in waiting:
+                if req.rid not in preps:
+                    preps[req.rid] = prepare_target(req)
+                occ = preps[req.rid][0]
+                box = tuple(padded_size(int(v)) for v in occ.shape)
+                b = self._bin_for(box)
+                q = b.free_slot() if b is not None else None
+                if q is None:
+                    still.append(req)
+                    continue
+                b.admit(req, preps.pop(req.rid), q, time.perf_counter() - t0)
+            waiting = still
+            busy = False
+            for b in list(self.bins.values()):
+                if not b.busy():
+                    continue
+                busy = True
+                if b.inflight:
+                    if not b.event.query():
+                        continue
+                    b.inflight = False
+                    res = b.collect(t0)
+                    results.extend(res)
+                    if log and res:
+                        log(f"t={time.perf_counter() - t0:.3f}s bin {b.box} done {[r.rid for r in res]}")
+                if b.busy():
+                    b.advance()
+            if not busy and not waiting and pi < len(pending):
+                time.sleep(max(0.0, min(0.002, pending[pi].arrival - (time.perf_counter() - t0))))
+        return results, time.perf_counter() - t0
+
+
+def latency_stats(results: List[Result]) -> dict:
+    lat = np.array(sorted(r.latency for r in results))
+    pct = lambda q: float(np.percentile(lat, q)) if len(lat) else float("nan")
+    return {"p50_ms": 1e3 * pct(50), "p95_ms": 1e3 * pct(95), "p99_ms": 1e3 * pct(99),
+            "mean_ms": 1e3 * float(lat.mean()) if len(lat) else float("nan")}

@@ -27,6 +27,31 @@ def test_green_table_matches_oracle(n):
     assert np.abs(g32 - ref).max() / np.abs(ref).max() < 1e-6
 
 
+def test_green_plan_matches_one_shot_build():
+    """A reused table plan gives the one-shot build's table for each k in turn."""
+    import ctypes
+    from precondda_amd._lib import check, lib, stream_ptr
+    n = (12, 10, 16)
+    tab = (n[0] + 1) * (n[1] + 1) * (n[2] + 1) * 6
+    gp = ctypes.c_void_p()
+    check(lib().pdda_green_plan_create(ctypes.byref(gp), *n))
+    try:
+        for k in (0.31, 0.17, 0.45):
+            a64 = torch.empty(tab, dtype=torch.complex128, device="cuda")
+            a32 = torch.empty(tab, dtype=torch.complex64, device="cuda")
+            b64 = torch.empty_like(a64)
+            check(lib().pdda_green_plan_build(gp, k, 1.0, ctypes.c_void_p(a64.data_ptr()),
+                                              ctypes.c_void_p(a32.data_ptr()), stream_ptr()))
+            check(lib().pdda_green_build(*n, k, 1.0, ctypes.c_void_p(b64.data_ptr()), None, stream_ptr()))
+            torch.cuda.synchronize()
+            assert torch.equal(a64, b64)
+            assert torch.equal(a32, b64.to(torch.complex64))
+            ref = O.generate_interaction_row(*n, k, 1.0).reshape(-1)
+            assert np.abs(a64.cpu().numpy() - ref).max() / np.abs(ref).max() < 1e-13
+    finally:
+        lib().pdda_green_plan_destroy(gp)
+
+
 @pytest.mark.parametrize("R", [3, 4, 8])
 def test_interaction_mult_matches_dense(R):
     import precondda_amd as P

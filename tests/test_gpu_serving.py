@@ -1,0 +1,109 @@
+"""Continuous-batching service: every result is the same alone and in the mix, and each
+one solves its own system (FP64 residual recomputed by the oracle on the trimmed box).
+
+Tolerance: with the tile autotuner off, every plan of a box size runs the same kernels,
+and a slot's arithmetic never reads another slot, so alone and in the mix agree to
+rounding of the reduction order: ||P_mix - P_alone|| / ||P_alone|| <= 1e-12.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import dda_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-6
+
+
+def _engine(**kw):
+    from precondda_amd.serving import ServingEngine
+    args = dict(hbm_budget=8e9, slots_per_bin=4, tol=TOL, check_every=8)
+    args.update(kw)
+    return ServingEngine(**args)
+
+
+def _by_rid(results):
+    return {r.rid: r for r in results}
+
+
+def test_alone_equals_mix(monkeypatch):
+    from precondda_amd.serving import make_requests
+    monkeypatch.setenv("PDDA_NO_AUTOTUNE", "1")
+    reqs = make_requests(6, seed=3, edges=(24, 32))
+    print("requests", [(r.rid, r.shape["type"], round(r.x, 3), r.m) for r in reqs], flush=True)
+    mix, _ = _engine().run(reqs)
+    assert len(mix) == len(reqs)
+    mix = _by_rid(mix)
+    for r in reqs:
+        alone, _ = _engine().run([r])
+        a = alone[0]
+        b = mix[r.rid]
+        assert a.ok and b.ok, (a.rel_residual, b.rel_residual)
+        assert a.grid == b.grid
+        d = float(torch.linalg.norm(a.P - b.P) / torch.linalg.norm(a.P))
+        print(r.rid, r.shape["type"], a.grid, "outer", a.outer, b.outer, "inner", a.inner_iters, b.inner_iters,
+              "diff", d, flush=True)
+        assert d <= 1e-12
+
+
+def test_late_arrivals_join_running_bin(monkeypatch):
+    """Requests arriving while others iterate are admitted into the same bin's free
+    slots; their results match the same requests solved alone."""
+    from precondda_amd.serving import make_requests
+    monkeypatch.setenv("PDDA_NO_AUTOTUNE", "1")
+    reqs = make_requests(4, seed=5, edges=(24,))
+    for q, r in enumerate(reqs):
+        r.arrival = 0.01 * q
+    mix, _ = _engine(slots_per_bin=2).run(reqs)
+    mix = _by_rid(mix)
+    assert len(mix) == 4
+    for r in reqs:
+        alone, _ = _engine(slots_per_bin=2).run([r])
+        d = float(torch.linalg.norm(alone[0].P - mix[r.rid].P) / torch.linalg.norm(alone[0].P))
+        print(r.rid, "diff", d, flush=True)
+        assert d <= 1e-12
+
+
+def test_results_solve_their_systems():
+    """FP64 residual of each served P, recomputed by the oracle on the trimmed box."""
+    from precondda_amd.serving import make_requests, prepare_target
+    reqs = make_requests(3, seed=7, edges=(24, 32))
+    res, _ = _engine(use_graph=True).run(reqs)
+    for r in sorted(res, key=lambda v: v.rid):
+        req = reqs[r.rid]
+        occ, nd, k, alpha = prepare_target(req)
+        occ = occ.cpu().numpy()
+        grid = occ.shape
+        mask = (occ != 0).astype(np.uint8)
+        ia = np.where(mask != 0, 1.0 / alpha, 0.0).astype(np.complex128)
+        G = O.generate_interaction_row(*grid, k, 1.0)
+        E = O.plane_wave(grid, k, req.kdir, req.epol)
+        rel = O.true_residual(r.P.cpu().numpy(), E, ia, mask, G, grid)
+        print(r.rid, grid, nd, "served", r.rel_residual, "oracle", rel, flush=True)
+        assert abs(k - r.k) < 1e-15 * k
+        assert rel < 1.05 * TOL
+
+
+def test_graph_equals_eager(monkeypatch):
+    from precondda_amd.serving import make_requests
+    monkeypatch.setenv("PDDA_NO_AUTOTUNE", "1")
+    reqs = make_requests(3, seed=11, edges=(24,))
+    g = _by_rid(_engine(use_graph=True).run(reqs)[0])
+    e = _by_rid(_engine(use_graph=False).run(reqs)[0])
+    for rid in g:
+        assert torch.equal(g[rid].P, e[rid].P)
+        assert g[rid].inner_iters == e[rid].inner_iters
+
+
+def test_latency_stats_and_budget():
+    from precondda_amd.serving import latency_stats, make_requests, slot_bytes, padded_size
+    reqs = make_requests(5, seed=2, edges=(24, 32), rate=200.0)
+    budget = 3 * slot_bytes((32, 32, 32))
+    eng = _engine(hbm_budget=budget, slots_per_bin=2)
+    res, wall = eng.run(reqs)
+    assert len(res) == 5 and all(r.ok for r in res)
+    st = latency_stats(res)
+    print(st, wall, flush=True)
+    assert 0 < st["p50_ms"] <= st["p95_ms"] <= st["p99_ms"] <= 1e3 * wall + 1e-6
+    assert all(r.done >= r.arrival for r in res)

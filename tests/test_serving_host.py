@@ -1,0 +1,55 @@
+"""Host logic of the solve service: request stream, padded box sizes, memory per slot."""
+import math
+
+import numpy as np
+import pytest
+
+
+def test_request_stream_distribution():
+    from precondda_amd.serving import make_requests
+    reqs = make_requests(200, seed=0, rate=50.0)
+    again = make_requests(200, seed=0, rate=50.0)
+    assert [(r.m, r.x, r.arrival) for r in reqs] == [(r.m, r.x, r.arrival) for r in again]
+    kinds = {r.shape["type"] for r in reqs}
+    assert kinds == {"sphere", "ellipsoid", "rectangle"}
+    t = [r.arrival for r in reqs]
+    assert all(b >= a for a, b in zip(t, t[1:]))
+    gaps = np.diff([0.0] + t)
+    assert 0.5 / 50 < gaps.mean() < 2.0 / 50
+    for r in reqs:
+        assert 1.2 <= r.m.real <= 2.0 and 1e-3 <= r.m.imag <= 0.1
+        assert 1.0 <= r.x <= 15.0
+        assert abs(np.linalg.norm(r.kdir) - 1) < 1e-12 and abs(np.linalg.norm(r.epol) - 1) < 1e-12
+        assert abs(np.dot(r.kdir, r.epol)) < 1e-12
+        s = r.shape
+        if s["type"] == "sphere":
+            L = 2 * s["radius"]
+            vol = 4 / 3 * math.pi * s["radius"] ** 3
+        elif s["type"] == "ellipsoid":
+            a = s["semi_axes"]
+            L = 2 * a[2]
+            vol = 4 / 3 * math.pi * a[0] * a[1] * a[2]
+            assert 1.0 <= a[2] / a[0] <= 3.0 + 1.0 / a[0] and a[0] >= 1
+        else:
+            dims = s["dimensions"]
+            L = dims[2]
+            vol = dims[0] * dims[1] * dims[2]
+            assert dims[0] % 2 == 0 and dims[0] >= 2
+        assert L in (24, 32, 48, 64, 96)
+        ae = (3 * vol / (4 * math.pi)) ** (1 / 3)
+        assert abs(r.m) * r.x / ae <= 0.5 + 1e-12
+
+
+def test_padded_sizes():
+    from precondda_amd.serving import PADDED_SIZES, padded_size
+    assert padded_size(1) == 8 and padded_size(24) == 24 and padded_size(25) == 32 and padded_size(97) == 128
+    assert all(2 * s <= 512 for s in PADDED_SIZES)
+    with pytest.raises(ValueError):
+        padded_size(257)
+
+
+def test_slot_bytes_scales_with_box():
+    from precondda_amd.serving import slot_bytes
+    a, b = slot_bytes((32, 32, 32)), slot_bytes((64, 64, 64))
+    assert 7.5 < b / a < 8.5
+    assert slot_bytes((96, 96, 96)) < 2e9
