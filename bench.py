@@ -43,6 +43,11 @@ def parse():
     ap.add_argument("--rate", type=float, default=40.0, help="serving: Poisson arrival rate (requests/s)")
     ap.add_argument("--requests", type=int, default=200, help="serving: number of requests")
     ap.add_argument("--slots", type=int, default=16, help="serving: slots per padded-box bin")
+    ap.add_argument("--check-every", type=int, default=16, help="serving: most inner iterations per launch")
+    ap.add_argument("--share", default="bin", choices=["bin", "slot"],
+                    help="serving: device time per round for each bin, or for each running slot")
+    ap.add_argument("--quantum-ms", type=float, default=4.0,
+                    help="serving: target GPU time per bin launch (time slicing); 0 = fixed check-every")
     ap.add_argument("--warm-requests", type=int, default=40,
                     help="serving: requests (another seed) served before the timed stream")
     ap.add_argument("--hbm-budget", type=float, default=128e9, help="serving: HBM budget for bins (bytes)")
@@ -292,7 +297,8 @@ def serving_main(args):
     from precondda_amd.serving import ServingEngine, latency_stats, make_requests, padded_size
 
     def engine():
-        return ServingEngine(hbm_budget=args.hbm_budget, slots_per_bin=args.slots, tol=args.tol, check_every=8)
+        return ServingEngine(hbm_budget=args.hbm_budget, slots_per_bin=args.slots, tol=args.tol,
+                             check_every=args.check_every, quantum_ms=args.quantum_ms or None, share=args.share)
 
     reqs = make_requests(args.requests, seed=0, rate=args.rate)
     eng = engine()

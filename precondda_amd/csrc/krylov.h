@@ -40,11 +40,16 @@ template <typename T> struct SolverT {
   int sidx = 0;
   long long host_iter = 0;
   bool lockstep = true;  // all slots started together: replacement only at iteration 1
-  hipGraphExec_t graph_exec = nullptr;
-  int graph_n = 0;
-  hipStream_t graph_stream = nullptr;
-  bool graph_replace = false;
-  int graph_sidx = 0;
+  // captured iteration graphs, keyed by what the captured launch sequence depends on
+  struct GraphEntry {
+    hipGraphExec_t exec = nullptr;
+    int n = 0;
+    hipStream_t stream = nullptr;
+    bool replace = false;
+    int sidx = 0;
+  };
+  static constexpr int kMaxGraphs = 8;
+  std::vector<GraphEntry> graphs;
   // bound operands (device pointers, plan precision)
   const C<T>* ghat = nullptr;
   long long ghat_stride = 0;

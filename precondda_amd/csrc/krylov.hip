@@ -332,8 +332,9 @@ int SolverT<T>::init(int nx_, int ny_, int nz_, int px_, int py_, int pz_, int n
 
 template <typename T>
 void SolverT<T>::release() {
-  if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-  graph_exec = nullptr;
+  for (GraphEntry& g : graphs)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  graphs.clear();
   conv_plan_free(&mv);
   if (precon) conv_plan_free(&pc);
   C<T>* bufs[] = {vs.b, vs.r, vs.rhat, vs.p, vs.x, vs.best, vs.w, vs.u, vs.z, vs.s0, vs.s1,
@@ -453,27 +454,37 @@ int SolverT<T>::iterate_graph(int n_iter, hipStream_t s) {
   if (n_iter <= 0) return PDDA_OK;
   if (n_iter % 2) { set_error("iterate_graph: n_iter must be even (s/s_prev ping-pong)"); return PDDA_ERR_ARG; }
   if (lockstep && host_iter < 2) return iterate(n_iter, s);  // replacement step runs eagerly
-  if (!graph_exec || graph_n != n_iter || graph_stream != s || graph_replace != (!lockstep) || graph_sidx != sidx) {
-    if (graph_exec) { (void)hipGraphExecDestroy(graph_exec); graph_exec = nullptr; }
+  const bool rep = !lockstep;
+  GraphEntry* ge = nullptr;
+  for (GraphEntry& g : graphs)
+    if (g.n == n_iter && g.stream == s && g.replace == rep && g.sidx == sidx) ge = &g;
+  if (!ge) {
+    if ((int)graphs.size() >= kMaxGraphs) {  // drop the oldest
+      (void)hipGraphExecDestroy(graphs.front().exec);
+      graphs.erase(graphs.begin());
+    }
     hipGraph_t gr = nullptr;
     const long long hi = host_iter;
     const int si = sidx;
     PDDA_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
     int rc = iterate(n_iter, s);
     hipError_t e = hipStreamEndCapture(s, &gr);
-    if (rc) { if (gr) (void)hipGraphDestroy(gr); return rc; }
-    if (e != hipSuccess) { set_error(std::string("graph capture: ") + hipGetErrorString(e)); return PDDA_ERR_HIP; }
-    e = hipGraphInstantiate(&graph_exec, gr, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(gr);
-    if (e != hipSuccess) { set_error(std::string("graph instantiate: ") + hipGetErrorString(e)); return PDDA_ERR_HIP; }
-    graph_n = n_iter;
-    graph_stream = s;
-    graph_replace = !lockstep;
-    graph_sidx = si;
     host_iter = hi;  // capture enqueued nothing
     sidx = si;
+    if (rc) { if (gr) (void)hipGraphDestroy(gr); return rc; }
+    if (e != hipSuccess) { set_error(std::string("graph capture: ") + hipGetErrorString(e)); return PDDA_ERR_HIP; }
+    GraphEntry g;
+    e = hipGraphInstantiate(&g.exec, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    if (e != hipSuccess) { set_error(std::string("graph instantiate: ") + hipGetErrorString(e)); return PDDA_ERR_HIP; }
+    g.n = n_iter;
+    g.stream = s;
+    g.replace = rep;
+    g.sidx = si;
+    graphs.push_back(g);
+    ge = &graphs.back();
   }
-  PDDA_HIP_CHECK(hipGraphLaunch(graph_exec, s));
+  PDDA_HIP_CHECK(hipGraphLaunch(ge->exec, s));
   host_iter += n_iter;
   return PDDA_OK;
 }

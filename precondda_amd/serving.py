@@ -160,7 +160,8 @@ class Bin:
     """One padded box size: a mixed solver with `cap` independent slots."""
 
     def __init__(self, box: Tuple[int, int, int], cap: int, tol: float, inner_target: float, max_inner: int,
-                 max_outer: int, check_every: int, use_graph: bool):
+                 max_outer: int, check_every: int, use_graph: bool, quantum_ms: Optional[float] = None,
+                 share: str = "bin"):
         self.box = tuple(box)
         self.cap = cap
         self.tol, self.inner_target, self.max_inner = tol, inner_target, max_inner
@@ -189,8 +190,18 @@ class Bin:
         self._gp = gp
         self.slots: List[Optional[dict]] = [None] * cap
         self.bytes = cap * slot_bytes(self.box) + int(lib().pdda_green_plan_bytes(gp))
-        self.event = torch.cuda.Event()
+        self.ev_start = torch.cuda.Event(enable_timing=True)
+        self.event = torch.cuda.Event(enable_timing=True)
         self.inflight = False  # iterations launched and not yet collected
+        # time slicing: iterations per launch sized so one launch takes about quantum_ms
+        # (launches of all bins share the device's few hardware queues)
+        # share "bin": every bin gets one quantum per round; "slot": a quantum per running
+        # slot, so each request gets about the same device time
+        self.quantum_ms = quantum_ms
+        self.share = share
+        self.n_iter = check_every
+        self.last_n = check_every
+        self.iter_ms = None
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -237,10 +248,28 @@ class Bin:
     def advance(self):
         """Launches check_every inner iterations of every running slot; non-blocking."""
         if self.busy():
-            check(lib().pdda_mixed_iterate(self._h, self.check_every, 1 if self.use_graph else 0,
+            self.ev_start.record(self.stream)
+            check(lib().pdda_mixed_iterate(self._h, self.n_iter, 1 if self.use_graph else 0,
                                            stream_ptr(self.stream)))
             self.event.record(self.stream)
+            self.last_n = self.n_iter
             self.inflight = True
+
+    def retune(self):
+        """After a finished launch: per-iteration time -> iterations per launch (a power
+        of two between 2 and check_every)."""
+        if self.quantum_ms is None:
+            return
+        t = self.ev_start.elapsed_time(self.event) / self.last_n
+        self.iter_ms = t if self.iter_ms is None else 0.5 * (self.iter_ms + t)
+        q = self.quantum_ms * (self.n_running() if self.share == "slot" else 1)
+        n = 2
+        while n * 2 <= self.check_every and n * 2 * self.iter_ms <= q:
+            n *= 2
+        self.n_iter = n
+
+    def n_running(self) -> int:
+        return sum(s is not None for s in self.slots)
 
     def ready(self) -> bool:
         """True when the launched iterations have finished on the device."""
@@ -293,14 +322,14 @@ class ServingEngine:
 
     def __init__(self, hbm_budget: float = 64e9, slots_per_bin: int = 8, tol: float = 1e-6,
                  inner_target: float = 1e-3, max_inner: int = 3000, max_outer: int = 12, check_every: int = 8,
-                 use_graph: bool = True):
+                 use_graph: bool = True, quantum_ms: Optional[float] = None, share: str = "bin"):
         _lib.require_gpu()
         if check_every % 2:
             raise ValueError("check_every must be even")
         self.budget = hbm_budget
         self.slots_per_bin = slots_per_bin
         self.kw = dict(tol=tol, inner_target=inner_target, max_inner=max_inner, max_outer=max_outer,
-                       check_every=check_every, use_graph=use_graph)
+                       check_every=check_every, use_graph=use_graph, quantum_ms=quantum_ms, share=share)
         self.bins: Dict[Tuple[int, int, int], Bin] = {}
         self.bin_create_s = 0.0  # host time spent creating bins (allocation, tile autotuning)
 
@@ -368,6 +397,7 @@ class ServingEngine:
                     if not b.event.query():
                         continue
                     b.inflight = False
+                    b.retune()
                     res = b.collect(t0)
                     results.extend(res)
                     if log and res:

@@ -49,13 +49,14 @@ def test_alone_equals_mix(monkeypatch):
 
 def test_late_arrivals_join_running_bin(monkeypatch):
     """Requests arriving while others iterate are admitted into the same bin's free
-    slots; their results match the same requests solved alone."""
+    slots (time-sliced launches, graphs of several lengths); their results match the
+    same requests solved alone."""
     from precondda_amd.serving import make_requests
     monkeypatch.setenv("PDDA_NO_AUTOTUNE", "1")
     reqs = make_requests(4, seed=5, edges=(24,))
     for q, r in enumerate(reqs):
         r.arrival = 0.01 * q
-    mix, _ = _engine(slots_per_bin=2).run(reqs)
+    mix, _ = _engine(slots_per_bin=2, quantum_ms=0.5, check_every=16).run(reqs)
     mix = _by_rid(mix)
     assert len(mix) == 4
     for r in reqs:
