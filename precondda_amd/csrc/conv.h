@@ -62,6 +62,7 @@ struct ConvPlanBase {
   int f_rows = 0;   // P1/P5 rows per tile (power of 2)
   int f_ncol = 0;   // P2/P4 columns per tile (power of 2)
   int f_wz = 0;     // P3 kz per tile (power of 2)
+  int f_xv = 0;     // P3 register-product kernel: columns per tile (0: LDS Stockham P3)
   size_t f_lds[5] = {0, 0, 0, 0, 0};
 };
 
@@ -87,6 +88,7 @@ struct ConvArgs {
 // Fast path (conv_fast.hip): chooses compile-time-planned kernels and their tiles for
 // the passes whose FFT length has a compile-time plan; launches one pass.
 void fast_setup(ConvPlanBase* p);
+bool fast_p3_usable(const ConvPlanBase* p, const ConvArgs& a);
 int fast_launch(ConvPlanBase* p, int pass, const ConvArgs& a, hipStream_t s);
 // Times the candidate tile shapes of every fast pass on the device (on a private
 // stream, with plan-owned scratch) and keeps the fastest. PDDA_NO_AUTOTUNE=1 skips it.

@@ -629,7 +629,7 @@ static int conv_apply_t(ConvPlanBase* p, const ConvArgs& a, hipStream_t s, hipEv
   }
   if (ev) PDDA_HIP_CHECK(hipEventRecord(ev[2], s));
   // P3
-  if (p->fast[2]) {
+  if (fast_p3_usable(p, a)) {
     fast_launch(p, 2, a, s);
   } else {
     const int nblk = g.nslots * (g.Ly / 2 + 1) * ceil_div(g.Lz, g.wz3);

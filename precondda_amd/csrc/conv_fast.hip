@@ -4,12 +4,15 @@
 //   P1/P5 (z): rows are contiguous in z, so the tile is staged through LDS on both
 //   sides to keep the HBM accesses contiguous.
 #include <algorithm>
+#include <cstdint>
+#include <cstdio>
 #include <cstdlib>
 #include <string>
 #include <vector>
 
 #include "conv.h"
 #include "fft_ct.h"
+#include "xconv_reg.h"
 
 namespace pdda {
 
@@ -345,8 +348,56 @@ template <typename T, int L, bool INV> void launch_axis(ConvPlanBase* p, const C
                      ax, (const C<T>*)p->tw[1], a.ctl, a.need);
 }
 
+int device_cu_count() {
+  static int n = 0;
+  if (!n) {
+    int dev = 0, v = 0;
+    n = (hipGetDevice(&dev) == hipSuccess &&
+         hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
+            ? v
+            : 256;
+  }
+  return n;
+}
+
+// The register-product P3 reads table entries as 16-byte vectors.
+bool xconv_aligned(const ConvArgs& a) {
+  return ((uintptr_t)a.table & 15) == 0 && (a.table_stride & 1) == 0;
+}
+
+template <typename T, int L, int V> void launch_x(ConvPlanBase* p, const ConvArgs& a, hipStream_t s) {
+  if constexpr (XPlan<L>::ok) {
+    const ConvGeom& g = p->g;
+    constexpr int NT = XTile<L, V>::NT;
+    const size_t lds = (size_t)XTile<L, V>::lds_elems() * sizeof(C<T>);
+    // workgroups resident at once on the device (persistent grid)
+    static const int per_cu = [&] {
+      int nb = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kx_xconv<T, L, V>, NT, lds) != hipSuccess || nb < 1)
+        nb = 1;
+      return nb;
+    }();
+    const long long ntiles = (long long)g.nslots * 2 * (g.Ly / 2 + 1) * ((g.Lz + V - 1) / V);
+    const long long resident = (long long)per_cu * device_cu_count();
+    long long nblk = std::min(ntiles, resident);
+    nblk = std::max(8LL, (nblk + 7) / 8 * 8);
+    const T scale = T(1.0 / ((double)g.Lx * g.Ly * g.Lz));
+    hipLaunchKernelGGL((kx_xconv<T, L, V>), dim3((unsigned)nblk), dim3(NT), lds, s, (C<T>*)p->work_b,
+                       (const C<T>*)a.table, a.table_stride, g, (const C<T>*)p->tw[0], a.ctl, a.need, scale);
+  }
+}
+
 template <typename T, int L> void launch_p3(ConvPlanBase* p, const ConvArgs& a, hipStream_t s) {
   const ConvGeom& g = p->g;
+  if (p->f_xv && xconv_aligned(a)) {
+    if constexpr (sizeof(T) == 4) {
+      if (p->f_xv == 8) return launch_x<T, L, 8>(p, a, s);
+      if (p->f_xv == 16) return launch_x<T, L, 16>(p, a, s);
+    } else {
+      if (p->f_xv == 4) return launch_x<T, L, 4>(p, a, s);
+      if (p->f_xv == 8) return launch_x<T, L, 8>(p, a, s);
+    }
+  }
   const int nblk = g.nslots * (g.Ly / 2 + 1) * ((g.Lz + p->f_wz - 1) / p->f_wz);
   const T scale = T(1.0 / ((double)g.Lx * g.Ly * g.Lz));
   hipLaunchKernelGGL((kf_xconv<T, L>), dim3(nblk), dim3(FFT_NT), p->f_lds[2], s, (C<T>*)p->work_b,
@@ -398,6 +449,65 @@ template <typename T> void launch_pass_L(ConvPlanBase* p, int pass, int L, const
   }
 }
 
+// Host mirror of XPlan: L = P * Q of the register-product P3 (0 if none).
+bool xplan_pq(int L, int* P, int* Q) {
+  switch (L) {
+    case 16: *P = 4; *Q = 4; return true;
+    case 24: *P = 3; *Q = 8; return true;
+    case 32: *P = 4; *Q = 8; return true;
+    case 48: *P = 6; *Q = 8; return true;
+    case 64: *P = 8; *Q = 8; return true;
+    case 96: *P = 6; *Q = 16; return true;
+    case 128: *P = 8; *Q = 16; return true;
+    case 192: *P = 12; *Q = 16; return true;
+    case 256: *P = 16; *Q = 16; return true;
+    default: return false;
+  }
+}
+
+size_t device_lds_limit() {
+  static size_t lim = 0;
+  if (!lim) {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
+      lim = (size_t)v;
+    else
+      lim = 64 * 1024;
+  }
+  return lim;
+}
+
+// Register-product P3 variants (columns per tile) that fit this plan's LDS and block size.
+std::vector<int> xconv_candidates(int L, int prec) {
+  std::vector<int> out;
+  int P = 0, Q = 0;
+  if (!xplan_pq(L, &P, &Q)) return out;
+  const size_t es = prec ? 16 : 8;
+  const int vs[2] = {prec ? 4 : 8, prec ? 8 : 16};
+  for (int V : vs) {
+    const size_t elems = 2 * (size_t)L + (size_t)V * 3 * P * (Q + 1);
+    if (3 * P * V > 1024 || elems * es > device_lds_limit()) continue;
+    out.push_back(V);
+  }
+  return out;
+}
+
+// PDDA_P3_KERNEL=stockham|register forces the P3 kernel family (both are exact
+// restatements of the same convolution; the choice is a speed matter).
+bool autotune_log() {
+  const char* e = getenv("PDDA_AUTOTUNE_LOG");
+  return e && e[0] == '1';
+}
+
+int p3_forced() {
+  const char* e = getenv("PDDA_P3_KERNEL");
+  if (!e) return 0;
+  if (e[0] == 's') return 1;
+  if (e[0] == 'r') return 2;
+  return 0;
+}
+
 }  // namespace
 
 void fast_setup(ConvPlanBase* p) {
@@ -443,6 +553,14 @@ void fast_setup(ConvPlanBase* p) {
       p->f_lds[2] = (size_t)(g.Lx + (size_t)g.Lx * 6 * w) * es;
       p->fast[2] = 1;
     }
+    p->f_xv = 0;
+    // the register-product kernel reads and writes x rows n < Lx / 2, so it needs Lx = 2 nx
+    const std::vector<int> xv = g.Lx == 2 * g.nx ? xconv_candidates(g.Lx, prec) : std::vector<int>();
+    if (!xv.empty() && p3_forced() != 1) {
+      p->f_xv = xv[0];
+      p->fast[2] = 1;
+    }
+    if (p3_forced() == 2 && xv.empty()) p->f_xv = 0;
   }
   if (p->fast[0]) p->nwg5 = g.nx * ((g.ny + p->f_rows - 1) / p->f_rows);
 }
@@ -485,9 +603,10 @@ int fast_autotune(ConvPlanBase* p) {
   const size_t tab_elems = (size_t)(g.Lx / 2 + 1) * (g.Ly / 2 + 1) * (g.Lz / 2 + 1) * 6;
   void* tab = nullptr;
   PDDA_HIP_CHECK(hipMalloc(&tab, tab_elems * es));
-  PDDA_HIP_CHECK(hipMemsetAsync(tab, 0, tab_elems * es, s));
-  PDDA_HIP_CHECK(hipMemsetAsync(p->work_a, 0, (size_t)g.a_stride * g.nslots * es, s));
-  PDDA_HIP_CHECK(hipMemsetAsync(p->work_b, 0, (size_t)g.b_stride * g.nslots * es, s));
+  // non-zero timing data (0x3c003c00 words: finite in both precisions)
+  PDDA_HIP_CHECK(hipMemsetD32Async((hipDeviceptr_t)tab, 0x3c003c00, tab_elems * es / 4, s));
+  PDDA_HIP_CHECK(hipMemsetD32Async((hipDeviceptr_t)p->work_a, 0x3c003c00, (size_t)g.a_stride * g.nslots * es / 4, s));
+  PDDA_HIP_CHECK(hipMemsetD32Async((hipDeviceptr_t)p->work_b, 0x3c003c00, (size_t)g.b_stride * g.nslots * es / 4, s));
   ConvArgs a;
   a.mode = MODE_FIELD;
   a.x = p->work_b;    // any 3N-per-slot buffer works as timing input
@@ -530,17 +649,37 @@ int fast_autotune(ConvPlanBase* p) {
   if (p->fast[2]) {
     int pmax = 1;
     while (pmax * 2 <= g.Lz) pmax *= 2;
-    const int passes[1] = {2};
+    // P3 candidates are timed inside the P1..P5 sequence that runs around them (its
+    // input comes straight from P2); passes without a fast kernel are left out
+    int passes[5], np = 0;
+    for (int q = 0; q < 5; ++q)
+      if (p->fast[q]) passes[np++] = q;
+    const int forced = p3_forced();
     float best = 1e30f;
-    int bestw = p->f_wz;
-    for (int w : pow2_candidates(g.Lx, p->prec, 6, pmax)) {
-      p->f_wz = w;
-      p->f_lds[2] = (size_t)(g.Lx + (size_t)g.Lx * 6 * w) * es;
-      const float t = time_passes(p, passes, 1, a, s, e0, e1, reps);
-      if (t < best) { best = t; bestw = w; }
+    int bestw = p->f_wz, bestx = p->f_xv;
+    if (forced != 2 && p->f_wz > 0) {
+      p->f_xv = 0;
+      for (int w : pow2_candidates(g.Lx, p->prec, 6, pmax)) {
+        p->f_wz = w;
+        p->f_lds[2] = (size_t)(g.Lx + (size_t)g.Lx * 6 * w) * es;
+        const float t = time_passes(p, passes, np, a, s, e0, e1, reps);
+        if (autotune_log()) fprintf(stderr, "[pdda autotune] P3 stockham wz=%d: %.2f us\n", w, 1e3 * t);
+        if (t < best) { best = t; bestw = w; bestx = 0; }
+      }
+    }
+    if (forced != 1 && g.Lx == 2 * g.nx) {
+      for (int v : xconv_candidates(g.Lx, p->prec)) {
+        p->f_xv = v;
+        const float t = time_passes(p, passes, np, a, s, e0, e1, reps);
+        if (autotune_log()) fprintf(stderr, "[pdda autotune] P3 register V=%d: %.2f us\n", v, 1e3 * t);
+        if (t < best) { best = t; bestx = v; }
+      }
     }
     p->f_wz = bestw;
-    p->f_lds[2] = (size_t)(g.Lx + (size_t)g.Lx * 6 * bestw) * es;
+    p->f_xv = bestx;
+    if (autotune_log()) fprintf(stderr, "[pdda autotune] P3 L=%d prec=%d best %.2f us (wz %d, xv %d)\n", g.Lx, p->prec,
+                                1e3 * best, bestw, bestx);
+    if (bestw > 0) p->f_lds[2] = (size_t)(g.Lx + (size_t)g.Lx * 6 * bestw) * es;
   }
   hipError_t err = hipStreamSynchronize(s);
   (void)hipFree(tab);
@@ -549,6 +688,10 @@ int fast_autotune(ConvPlanBase* p) {
   (void)hipStreamDestroy(s);
   if (err != hipSuccess) { set_error(std::string("autotune: ") + hipGetErrorString(err)); return PDDA_ERR_HIP; }
   return PDDA_OK;
+}
+
+bool fast_p3_usable(const ConvPlanBase* p, const ConvArgs& a) {
+  return p->fast[2] && (p->f_wz > 0 || (p->f_xv > 0 && xconv_aligned(a)));
 }
 
 int fast_launch(ConvPlanBase* p, int pass, const ConvArgs& a, hipStream_t s) {

@@ -1,0 +1,256 @@
+// Pass P3 of the convolution as a four-step FFT whose middle step stays in registers.
+//
+// For one (ky, kz) column of the y/z-transformed grid, the x axis holds L = 2 nx
+// positions of which the first nx are non-zero; the pass computes
+//   out[ix] = IFFT_x( G(kx, ky, kz) . FFT_x(in) )[ix],  ix < nx,
+// for the 3 field components at once. With L = P Q, n = n1 + P n2, k = k1 + Q k2:
+//   step A  (thread per component, n1): Q-point DFT over n2 (only n2 < Q/2 non-zero),
+//           times w_L^(n1 k1), into LDS row [n1][k1];
+//   step B  (thread per k1, all 3 components): P-point DFTs over n1 give the Fourier
+//           coefficients kx = k1 + Q k2, k2 < P, in registers; the 3x3 symmetric
+//           Green's product; inverse P-point DFTs over k2 times w_L^(-n1 k1), written
+//           back to the same LDS column;
+//   step C  (thread per component, n1): inverse Q-point DFT over k1; only n2 < Q/2
+//           (ix < nx) is computed and stored.
+// Two LDS exchanges per column, no reordering: the product is applied in the order the
+// coefficients come out of step B. The 1/(Lx Ly Lz) of the inverse transform rides on
+// the step-A twiddles.
+#pragma once
+
+#include "fft_ct.h"
+
+namespace pdda {
+
+// a * exp(-2 pi i m / N), with exact quarter turns and eighth roots.
+template <typename T, int N> __device__ inline C<T> cmul_root(C<T> a, int m) {
+  m %= N;
+  const T c = T(0.70710678118654752440);
+  if (m == 0) return a;
+  if (4 * m == N) return cmul_mi(a);
+  if (2 * m == N) return mk<T>(-a.x, -a.y);
+  if (4 * m == 3 * N) return mk<T>(-a.y, a.x);
+  if (8 * m == N) return mk<T>(c * (a.x + a.y), c * (a.y - a.x));
+  if (8 * m == 3 * N) return mk<T>(c * (a.y - a.x), -c * (a.x + a.y));
+  if (8 * m == 5 * N) return mk<T>(-c * (a.x + a.y), c * (a.x - a.y));
+  if (8 * m == 7 * N) return mk<T>(c * (a.x - a.y), c * (a.x + a.y));
+  return cmul(a, mk<T>(T(root_re<N>(m)), T(root_im<N>(m))));
+}
+
+// Forward DFT of N registers whose inputs n >= N/2 are zero (they are not read):
+// X[2k] = DFT_{N/2}(x)[k], X[2k+1] = DFT_{N/2}(x[n] w_N^n)[k].
+template <typename T, int N> __device__ inline void dft_lowhalf(C<T>* v) {
+  constexpr int H = N / 2;
+  C<T> e[H], o[H];
+#pragma unroll
+  for (int n = 0; n < H; ++n) {
+    e[n] = v[n];
+    o[n] = cmul_root<T, N>(v[n], n);
+  }
+  DftC<T, H>::run(e);
+  DftC<T, H>::run(o);
+#pragma unroll
+  for (int k = 0; k < H; ++k) {
+    v[2 * k] = e[k];
+    v[2 * k + 1] = o[k];
+  }
+}
+
+// Factorisations L = P * Q of the register-product P3 (Q a power of two, so that the
+// zero half of the input is a whole half of each step-A DFT).
+template <int L> struct XPlan { static constexpr bool ok = false; static constexpr int P = 1, Q = 2; };
+#define PDDA_XPLAN(Lv, Pv, Qv)                \
+  template <> struct XPlan<Lv> {              \
+    static constexpr bool ok = true;          \
+    static constexpr int P = Pv, Q = Qv;      \
+  };
+PDDA_XPLAN(16, 4, 4)
+PDDA_XPLAN(24, 3, 8)
+PDDA_XPLAN(32, 4, 8)
+PDDA_XPLAN(48, 6, 8)
+PDDA_XPLAN(64, 8, 8)
+PDDA_XPLAN(96, 6, 16)
+PDDA_XPLAN(128, 8, 16)
+PDDA_XPLAN(192, 12, 16)
+PDDA_XPLAN(256, 16, 16)
+#undef PDDA_XPLAN
+
+template <int L, int V> struct XTile {
+  static constexpr int P = XPlan<L>::P, Q = XPlan<L>::Q;
+  static constexpr int NT = 3 * P * V;      // threads (steps A and C)
+  static constexpr int RS = Q + 1;          // LDS row stride (odd: spreads banks)
+  static constexpr int HX = L / 2;
+  static constexpr int lds_elems() { return 2 * L + V * 3 * P * RS; }
+};
+
+// The six reduced-table components of one (mx, my, mz) entry as 16-byte loads.
+template <typename T> __device__ inline void load_entry(const C<T>* __restrict__ p, C<T> (&g)[6]) {
+  if constexpr (sizeof(T) == 4) {
+    const float4* q = reinterpret_cast<const float4*>(p);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const float4 v = q[i];
+      g[2 * i] = make_float2(v.x, v.y);
+      g[2 * i + 1] = make_float2(v.z, v.w);
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) g[i] = p[i];
+  }
+}
+
+// B: (slot, ix < nx, ky < Ly, kz < Lz, 3) after the y/z forward transforms, updated in
+// place. A tile is one ky row and V consecutive kz (every x row of the tile is one run
+// of 3 V elements). Persistent workgroups: the grid is what the device holds at once
+// (a multiple of 8); the tiles, ordered slot fastest, then ky and its mirror Ly - ky,
+// then the kz block interleaved with its mirror block, then the ky pair -- so that
+// tiles reading the same reduced-table rows are neighbours -- are split into 8
+// contiguous ranges, one per XCD (one L2). Each workgroup prefetches the next tile's
+// inputs into registers while it transforms the current one. The table is read from
+// L2 in step B, six components per 16-byte-aligned entry.
+template <typename T, int L, int V>
+__global__ __launch_bounds__(3 * XPlan<L>::P * V) void kx_xconv(C<T>* __restrict__ B, const C<T>* __restrict__ table,
+                                                            long long table_stride, ConvGeom g,
+                                                            const C<T>* __restrict__ twx, const SlotCtl* ctl,
+                                                            int need, T scale) {
+  using X = XTile<L, V>;
+  constexpr int P = X::P, Q = X::Q, NT = X::NT, RS = X::RS, HX = X::HX;
+  extern __shared__ __align__(16) unsigned char smem[];
+  C<T>* twa = reinterpret_cast<C<T>*>(smem);  // scale * w^(n k1), step A
+  C<T>* twb = twa + L;                        // w^(n k1), step B
+  C<T>* dat = twb + L;
+
+  const int hy = g.Ly / 2, hz = g.Lz / 2;
+  const int nkzb = (g.Lz + V - 1) / V;
+  const int ntiles = g.nslots * 2 * nkzb * (hy + 1);
+  const int xcd = blockIdx.x & 7, nper = gridDim.x >> 3;
+  const int per = ntiles >> 3, remt = ntiles & 7;
+  const int tbeg = xcd * per + min(xcd, remt), tend = tbeg + per + (xcd < remt ? 1 : 0);
+  const long long W = 3LL * g.Lz;
+  const long long xs = (long long)g.Ly * W;
+  const long long mxs = (long long)(hy + 1) * (hz + 1) * 6;
+  const int t = threadIdx.x;
+  // step A/C thread roles: component fastest, then kz, then n1 (row t of `dat`)
+  const int ca = t % 3, qa = (t / 3) % V, n1a = t / (3 * V);
+
+  struct Tile {
+    int slot, qy, ky, kz0;
+    bool ok;
+  };
+  auto decode = [&](int lt) {
+    Tile u;
+    u.ok = lt < tend;
+    if (!u.ok) return u;
+    u.slot = lt % g.nslots;
+    int r = lt / g.nslots;
+    const int yr = r & 1;
+    r >>= 1;
+    const int kbp = r % nkzb;
+    u.qy = r / nkzb;
+    u.kz0 = ((kbp & 1) ? nkzb - 1 - (kbp >> 1) : (kbp >> 1)) * V;
+    u.ky = yr ? g.Ly - u.qy : u.qy;
+    u.ok = !(yr && (u.qy == 0 || u.qy == hy)) && slot_wanted(ctl, u.slot, need);
+    return u;
+  };
+  // this thread's step-A/C element offset within B for a tile (-1: not live)
+  auto offset_a = [&](const Tile& u) -> long long {
+    if (!u.ok || u.kz0 + qa >= g.Lz) return -1;
+    return u.slot * g.b_stride + (long long)u.ky * W + 3LL * (u.kz0 + qa) + ca + (long long)n1a * xs;
+  };
+  auto load_inputs = [&](const Tile& u, C<T> (&zz)[Q / 2]) {
+    const long long o = offset_a(u);
+#pragma unroll
+    for (int n2 = 0; n2 < Q / 2; ++n2) zz[n2] = o >= 0 ? B[o + (long long)(n2 * P) * xs] : mk<T>(T(0), T(0));
+  };
+
+  for (int i = t; i < L; i += NT) {
+    const C<T> w = twx[(i / Q) * (i % Q)];
+    twa[i] = cscale(w, scale);
+    twb[i] = w;
+  }
+
+  int lt = tbeg + (blockIdx.x >> 3);
+  Tile cur = decode(lt);
+  C<T> zc[Q / 2];
+  load_inputs(cur, zc);
+  while (lt < tend) {
+    const int ln = lt + nper;
+    const Tile nxt = decode(ln);
+    // orders the twiddle writes, and the previous tile's step C reads of `dat`, before
+    // this tile's step A writes
+    __syncthreads();
+    if (cur.ok) {
+      // step A: Q-point DFT with the upper half zero, times scale w^(n1 k1)
+      C<T> z[Q];
+#pragma unroll
+      for (int n2 = 0; n2 < Q / 2; ++n2) z[n2] = zc[n2];
+      dft_lowhalf<T, Q>(z);
+      C<T>* row = dat + t * RS;
+#pragma unroll
+      for (int k1 = 0; k1 < Q; ++k1) row[k1] = cmul(z[k1], twa[n1a * Q + k1]);
+    }
+    // next tile's inputs in flight during steps B and C
+    load_inputs(nxt, zc);
+    __syncthreads();
+    if (cur.ok && t < Q * V) {
+      // step B: per (kz, k1), all three components in registers
+      const int k1 = t % Q, v = t / Q;
+      const int kz = cur.kz0 + v;
+      if (kz < g.Lz) {
+        C<T> xv[3][P];
+        C<T>* col = dat + (v * 3) * RS + k1;  // rows (n, v, c) = (n * V + v) * 3 + c
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int n = 0; n < P; ++n) xv[c][n] = col[(n * V * 3 + c) * RS];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) DftC<T, P>::run(xv[c]);
+        const bool fz = kz > hz, fy = cur.ky > hy;
+        const int mz = fz ? g.Lz - kz : kz;
+        const C<T>* tg = table + cur.slot * table_stride + ((long long)cur.qy * (hz + 1) + mz) * 6;
+#pragma unroll
+        for (int k2 = 0; k2 < P; ++k2) {
+          const int kx = k1 + Q * k2;
+          const bool fx = kx > HX;
+          const int mx = fx ? L - kx : kx;
+          C<T> ge[6];
+          load_entry<T>(tg + mx * mxs, ge);
+          C<T> h1 = ge[1], h2 = ge[2], h4 = ge[4];
+          if (fx != fy) h1 = mk<T>(-h1.x, -h1.y);
+          if (fx != fz) h2 = mk<T>(-h2.x, -h2.y);
+          if (fy != fz) h4 = mk<T>(-h4.x, -h4.y);
+          const C<T> a0 = xv[0][k2], a1 = xv[1][k2], a2 = xv[2][k2];
+          // inverse transforms below use conj(DFT(conj(.))): keep the conjugate
+          xv[0][k2] = cconj(cadd(cadd(cmul(ge[0], a0), cmul(h1, a1)), cmul(h2, a2)));
+          xv[1][k2] = cconj(cadd(cadd(cmul(h1, a0), cmul(ge[3], a1)), cmul(h4, a2)));
+          xv[2][k2] = cconj(cadd(cadd(cmul(h2, a0), cmul(h4, a1)), cmul(ge[5], a2)));
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) DftC<T, P>::run(xv[c]);
+        // conj(result) * conj(w^(n1 k1)) = conj(result * w^(n1 k1)); step C conjugates
+        // its input again, so the rows keep the conjugated values
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int n = 0; n < P; ++n) col[(n * V * 3 + c) * RS] = n == 0 ? xv[c][0] : cmul(xv[c][n], twb[n * Q + k1]);
+      }
+    }
+    __syncthreads();
+    {
+      // step C: inverse Q-point DFT over k1 (rows hold conjugated values), ix < nx
+      const long long o = offset_a(cur);
+      if (o >= 0) {
+        const C<T>* row = dat + t * RS;
+        C<T> z[Q];
+#pragma unroll
+        for (int k1 = 0; k1 < Q; ++k1) z[k1] = row[k1];
+        DftC<T, Q>::run(z);
+#pragma unroll
+        for (int n2 = 0; n2 < Q / 2; ++n2) B[o + (long long)(n2 * P) * xs] = cconj(z[n2]);
+      }
+    }
+    lt = ln;
+    cur = nxt;
+  }
+}
+
+}  // namespace pdda
