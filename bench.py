@@ -207,7 +207,8 @@ def main():
         ms5 = plan.time_passes(_lib.PDDA_MODE_INTERACT, x, out, G, ia, mask, reps=50)
         byts = matvec_bytes(N, Np, e)
         t_mv = sum(ms5) * 1e-3
-        mv[name] = {"ms_per_pass": [round(v, 5) for v in ms5], "ms": round(sum(ms5), 5),
+        mv[name] = {"kernels": [plan.describe(q) for q in range(5)],
+                    "ms_per_pass": [round(v, 5) for v in ms5], "ms": round(sum(ms5), 5),
                     "GBps": round(sum(byts) / t_mv / 1e9, 1), "frac_of_peak": round(sum(byts) / t_mv / peak, 4),
                     "bytes": int(sum(byts)),
                     "pass_GBps": [round(bb / (m * 1e-3) / 1e9, 1) for bb, m in zip(byts, ms5)]}
@@ -220,18 +221,33 @@ def main():
         ms5 = plan.time_passes(_lib.PDDA_MODE_INTERACT, xb, outb, G32, ia32, mask, reps=10)
         bb = (82 * N * nrhs + 6 * Np) * 8
         t_mv = sum(ms5) * 1e-3
-        mv["c64_block"] = {"rhs": nrhs, "ms_per_pass": [round(v, 5) for v in ms5], "ms": round(sum(ms5), 5),
+        mv["c64_block"] = {"rhs": nrhs, "kernels": [plan.describe(q) for q in range(5)],
+                           "ms_per_pass": [round(v, 5) for v in ms5], "ms": round(sum(ms5), 5),
                            "GBps": round(bb / t_mv / 1e9, 1), "frac_of_peak": round(bb / t_mv / peak, 4),
                            "bytes": int(bb)}
     log(f"matvec {mv}")
     dom = int(np.argmax(mv["c64"]["ms_per_pass"]))
-    names = ["k_zfwd", "k_axis(y-fwd)", "k_xconv", "k_axis(y-inv)", "k_zinv"]
     byts64 = matvec_bytes(N, Np, 8)
-    roofline = {"bound": "hbm", "kernel": names[dom] + " complex64", "achieved": round(byts64[dom] / (mv["c64"]["ms_per_pass"][dom] * 1e-3) / 1e9, 1),
+    roofline = {"bound": "hbm", "kernel": mv["c64"]["kernels"][dom] + ", one RHS", "achieved": round(byts64[dom] / (mv["c64"]["ms_per_pass"][dom] * 1e-3) / 1e9, 1),
                 "peak": 8000.0, "unit": "GB/s",
                 "frac": round(byts64[dom] / (mv["c64"]["ms_per_pass"][dom] * 1e-3) / peak, 4),
                 "traffic": None, "matvec_GBps_c64": mv["c64"]["GBps"], "matvec_frac_c64": mv["c64"]["frac_of_peak"],
                 "matvec_GBps_c128": mv["c128"]["GBps"], "matvec_frac_c128": mv["c128"]["frac_of_peak"]}
+
+    # HBM traffic of the roofline kernel per launch, from the rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE runs of this workload committed under profiles/ (used only when the
+    # same kernel was chosen here)
+    try:
+        with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                               "matvec_64cube_traffic.json")) as f:
+            tr = json.load(f)
+        kname = mv["c64"]["kernels"][dom]
+        if tuple(tr["grid"]) == tuple(wl.grid) and kname in tr["kernels"]:
+            roofline["traffic"] = tr["kernels"][kname]["traffic_bytes"]
+            roofline["traffic_unit"] = "bytes per launch (rocprofv3 FETCH_SIZE x%.0f + WRITE_SIZE)" % tr["fetch_factor"]
+            roofline["algorithmic_bytes"] = int(byts64[dom])
+    except (OSError, ValueError, KeyError):
+        pass
 
     # ---------------- CPU baseline (oracle restatement, bounded sample, extrapolated)
     cpu = None

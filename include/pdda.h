@@ -96,6 +96,9 @@ int pdda_precond_build(int hx, int hy, int hz, double gamma_re, double gamma_im,
 int pdda_conv_create(pdda_conv** out, int prec, int nx, int ny, int nz, int hx, int hy, int hz, int nslots);
 int pdda_conv_destroy(pdda_conv* c);
 long long pdda_conv_workspace_bytes(const pdda_conv* c);
+/* Names the kernel (and its tile) each pass of the plan launches: pass 0..4 = z-forward,
+ * y-forward, x-convolution, y-inverse, z-inverse. No counterpart in the reference. */
+int pdda_conv_describe(const pdda_conv* c, int pass, char* buf, int n);
 /* One convolution per slot. Replaces fft_matvec + interaction_mult
  * (Gpbicgstab.py:742-772, 828-830), fft_matvec3 (:633-668) and the product of
  * E_field_solver (ExcitedField.py:155-195). table_stride / site_stride: elements

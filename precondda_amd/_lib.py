@@ -49,6 +49,7 @@ _SIGS = {
     "pdda_conv_create": (_i, [_c.POINTER(_vp), _i, _i, _i, _i, _i, _i, _i, _i]),
     "pdda_conv_destroy": (_i, [_vp]),
     "pdda_conv_workspace_bytes": (_ll, [_vp]),
+    "pdda_conv_describe": (_i, [_vp, _i, _c.c_char_p, _i]),
     "pdda_conv_apply": (_i, [_vp, _i, _vp, _vp, _vp, _ll, _vp, _vp, _ll, _vp, _vp]),
     "pdda_conv_time_passes": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _i, _c.POINTER(_d), _vp]),
     "pdda_dot": (_i, [_i, _vp, _vp, _ll, _c.POINTER(_d), _vp]),

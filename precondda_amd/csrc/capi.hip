@@ -1,7 +1,9 @@
 // extern "C" boundary of libpdda.so (declared in include/pdda.h).
 #include "../../include/pdda.h"
 
+#include <cstdio>
 #include <new>
+#include <string>
 
 #include "krylov.h"
 
@@ -215,6 +217,30 @@ long long pdda_conv_workspace_bytes(const pdda_conv* c) {
   if (!c) return 0;
   const size_t es = c->p.prec ? sizeof(double2) : sizeof(float2);
   return (long long)((c->p.g.a_stride + c->p.g.b_stride) * c->p.g.nslots * es);
+}
+
+int pdda_conv_describe(const pdda_conv* c, int pass, char* buf, int n) {
+  if (!c || !buf || n <= 0 || pass < 0 || pass > 4) { set_error("conv_describe: bad argument"); return PDDA_ERR_ARG; }
+  const ConvPlanBase& p = c->p;
+  const char* T = p.prec ? "double" : "float";
+  const int L = (pass == 0 || pass == 4) ? p.g.Lz : (pass == 2 ? p.g.Lx : p.g.Ly);
+  std::string d;
+  if (!p.fast[pass]) {
+    static const char* gen[5] = {"k_zfwd", "k_axis", "k_xconv", "k_axis", "k_zinv"};
+    d = std::string(gen[pass]) + "<" + T + "> (runtime plan, L=" + std::to_string(L) + ")";
+  } else if (pass == 2 && p.f_xv) {
+    d = std::string("kx_xconv<") + T + ", " + std::to_string(L) + ", " + std::to_string(p.f_xv) + ">";
+  } else if (pass == 2) {
+    d = std::string("kf_xconv<") + T + ", " + std::to_string(L) + "> (wz " + std::to_string(p.f_wz) + ")";
+  } else if (pass == 1 || pass == 3) {
+    d = std::string("kf_axis<") + T + ", " + std::to_string(L) + ", " + (pass == 3 ? "true" : "false") + "> (cols " +
+        std::to_string(p.f_ncol) + ")";
+  } else {
+    d = std::string(pass == 0 ? "kf_zfwd<" : "kf_zinv<") + T + ", " + std::to_string(L) + "> (rows " +
+        std::to_string(p.f_rows) + ")";
+  }
+  std::snprintf(buf, (size_t)n, "%s", d.c_str());
+  return PDDA_OK;
 }
 
 int pdda_conv_apply(pdda_conv* c, int mode, const void* x, void* out, const void* table, long long table_stride,

@@ -92,6 +92,12 @@ class ConvPlan:
                                           int(reps), ms, stream_ptr(stream)))
         return list(ms)
 
+    def describe(self, pass_index: int) -> str:
+        """Kernel and tile that pass 0..4 launches (after tile autotuning)."""
+        buf = ctypes.create_string_buffer(160)
+        check(lib().pdda_conv_describe(self._h, int(pass_index), buf, 160))
+        return buf.value.decode()
+
     def workspace_bytes(self) -> int:
         return int(lib().pdda_conv_workspace_bytes(self._h))
 
