@@ -86,6 +86,8 @@ def make_requests(n: int = 200, seed: int = 0, rate: Optional[float] = None,
     L resampled when that bound is below 1), random direction and polarisation angle.
     Arrivals: Poisson with `rate` per second (all at t = 0 when rate is None)."""
     rng = np.random.default_rng(seed)
+    # arrival times from their own stream: the same requests whatever the rate
+    rng_t = np.random.default_rng([seed, 1])
     reqs: List[Request] = []
     t = 0.0
     kinds = ["sphere", "ellipsoid", "rectangle"]
@@ -120,7 +122,7 @@ def make_requests(n: int = 200, seed: int = 0, rate: Optional[float] = None,
         psi = float(rng.uniform(0.0, 2.0 * math.pi))
         ep = math.cos(psi) * e1 + math.sin(psi) * e2
         if rate is not None:
-            t += float(rng.exponential(1.0 / rate))
+            t += float(rng_t.exponential(1.0 / rate))
         reqs.append(Request(rid, shape, m, x, kd, ep, t if rate is not None else 0.0))
     return reqs
 
@@ -154,6 +156,12 @@ def slot_bytes(box) -> int:
     refine = 3 * 3 * N * c128 + 2 * 3 * N * c64 + 18 * N * c128  # b, x, r, e, d + c128 workspace
     operands = 6 * Np * (c64 + c128) + N * (c64 + c128 + 1) + 3 * N * c128
     return inner + refine + operands
+
+
+def green_plan_bytes(box) -> int:
+    """FP64 FFT workspace of a bin's table plan (two partial transforms of the 2n grid)."""
+    nx, ny, nz = box
+    return (2 * nx * 2 * ny + 2 * nx * (ny + 1)) * 6 * (nz + 1) * 16
 
 
 class Bin:
@@ -268,6 +276,9 @@ class Bin:
             n *= 2
         self.n_iter = n
 
+    def oldest_arrival(self) -> float:
+        return min((s["req"].arrival for s in self.slots if s is not None), default=math.inf)
+
     def n_running(self) -> int:
         return sum(s is not None for s in self.slots)
 
@@ -322,7 +333,8 @@ class ServingEngine:
 
     def __init__(self, hbm_budget: float = 64e9, slots_per_bin: int = 8, tol: float = 1e-6,
                  inner_target: float = 1e-3, max_inner: int = 3000, max_outer: int = 12, check_every: int = 8,
-                 use_graph: bool = True, quantum_ms: Optional[float] = None, share: str = "bin"):
+                 use_graph: bool = True, quantum_ms: Optional[float] = None, share: str = "bin",
+                 prep_ahead: int = 16, max_active: int = 0):
         _lib.require_gpu()
         if check_every % 2:
             raise ValueError("check_every must be even")
@@ -332,6 +344,8 @@ class ServingEngine:
                        check_every=check_every, use_graph=use_graph, quantum_ms=quantum_ms, share=share)
         self.bins: Dict[Tuple[int, int, int], Bin] = {}
         self.bin_create_s = 0.0  # host time spent creating bins (allocation, tile autotuning)
+        self.prep_ahead = prep_ahead  # requests marked ahead of a free slot
+        self.max_active = max_active  # bins in flight at once (0: all)
 
     def used_bytes(self) -> int:
         return sum(b.bytes for b in self.bins.values())
@@ -340,7 +354,7 @@ class ServingEngine:
         b = self.bins.get(box)
         if b is not None:
             return b
-        need = self.slots_per_bin * slot_bytes(box)
+        need = self.slots_per_bin * slot_bytes(box) + green_plan_bytes(box)
         if self.used_bytes() + need > self.budget:
             # release idle bins of other sizes to make room
             for key in [k for k, v in self.bins.items() if not v.busy()]:
@@ -348,7 +362,7 @@ class ServingEngine:
                 if self.used_bytes() + need <= self.budget:
                     break
         if self.used_bytes() + need > self.budget:
-            cap = int((self.budget - self.used_bytes()) // slot_bytes(box))
+            cap = int((self.budget - self.used_bytes() - green_plan_bytes(box)) // slot_bytes(box))
             if cap < 1:
                 return None
         else:
@@ -365,46 +379,62 @@ class ServingEngine:
         iterations have finished is collected (refinement, results, reloads) and relaunched
         at once, so bins of different sizes progress at their own pace."""
         pending = sorted(requests, key=lambda r: r.arrival)
-        preps = {}
         results: List[Result] = []
-        waiting: List[Request] = []
+        arrived: List[Request] = []            # not yet marked on the device (FIFO)
+        ready: List[Tuple[Request, tuple]] = []  # marked, waiting for a slot (FIFO)
         t0 = time.perf_counter()
         pi = 0
-        while pi < len(pending) or waiting or any(b.busy() for b in self.bins.values()):
+        while pi < len(pending) or arrived or ready or any(b.busy() for b in self.bins.values()):
             now = time.perf_counter() - t0
             while pi < len(pending) and pending[pi].arrival <= now:
-                waiting.append(pending[pi])
+                arrived.append(pending[pi])
                 pi += 1
+            # mark and trim only a few requests ahead of admission
+            while arrived and len(ready) < self.prep_ahead:
+                req = arrived.pop(0)
+                ready.append((req, prepare_target(req)))
             still = []
-            for req in waiting:
-                if req.rid not in preps:
-                    preps[req.rid] = prepare_target(req)
-                occ = preps[req.rid][0]
-                box = tuple(padded_size(int(v)) for v in occ.shape)
+            for req, prep in ready:
+                box = tuple(padded_size(int(v)) for v in prep[0].shape)
                 b = self._bin_for(box)
                 q = b.free_slot() if b is not None else None
                 if q is None:
-                    still.append(req)
+                    still.append((req, prep))
                     continue
-                b.admit(req, preps.pop(req.rid), q, time.perf_counter() - t0)
-            waiting = still
+                b.admit(req, prep, q, time.perf_counter() - t0)
+            ready = still
             busy = False
+            # collect finished launches
             for b in list(self.bins.values()):
                 if not b.busy():
                     continue
                 busy = True
-                if b.inflight:
-                    if not b.event.query():
-                        continue
+                if b.inflight and b.event.query():
                     b.inflight = False
                     b.retune()
                     res = b.collect(t0)
                     results.extend(res)
                     if log and res:
                         log(f"t={time.perf_counter() - t0:.3f}s bin {b.box} done {[r.rid for r in res]}")
-                if b.busy():
-                    b.advance()
-            if not busy and not waiting and pi < len(pending):
+            # relaunch: at most max_active bins in flight, oldest request first (fewer
+            # working sets in the device caches at once)
+            live = [b for b in self.bins.values() if b.busy()]
+            n_inflight = sum(b.inflight for b in live)
+            if self.max_active:
+                live.sort(key=lambda b: b.oldest_arrival())
+            for b in live:
+                if b.inflight:
+                    continue
+                if self.max_active and n_inflight >= self.max_active:
+                    break
+                b.advance()
+                n_inflight += 1
+            if not busy and ready:
+                # nothing runs, so every idle bin was released: this request alone
+                # does not fit the budget
+                req = ready[0][0]
+                raise RuntimeError(f"request {req.rid} needs more than the HBM budget ({self.budget:.3g} bytes)")
+            if not busy and not ready and not arrived and pi < len(pending):
                 time.sleep(max(0.0, min(0.002, pending[pi].arrival - (time.perf_counter() - t0))))
         return results, time.perf_counter() - t0
 

@@ -10,6 +10,9 @@ def test_request_stream_distribution():
     reqs = make_requests(200, seed=0, rate=50.0)
     again = make_requests(200, seed=0, rate=50.0)
     assert [(r.m, r.x, r.arrival) for r in reqs] == [(r.m, r.x, r.arrival) for r in again]
+    same = make_requests(200, seed=0, rate=None)
+    assert [(r.m, r.x, r.shape) for r in reqs] == [(r.m, r.x, r.shape) for r in same]
+    assert all(r.arrival == 0.0 for r in same)
     kinds = {r.shape["type"] for r in reqs}
     assert kinds == {"sphere", "ellipsoid", "rectangle"}
     t = [r.arrival for r in reqs]
