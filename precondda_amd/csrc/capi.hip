@@ -229,7 +229,8 @@ int pdda_conv_describe(const pdda_conv* c, int pass, char* buf, int n) {
     static const char* gen[5] = {"k_zfwd", "k_axis", "k_xconv", "k_axis", "k_zinv"};
     d = std::string(gen[pass]) + "<" + T + "> (runtime plan, L=" + std::to_string(L) + ")";
   } else if (pass == 2 && p.f_xv) {
-    d = std::string("kx_xconv<") + T + ", " + std::to_string(L) + ", " + std::to_string(p.f_xv) + ">";
+    d = std::string("kx_xconv<") + T + ", " + std::to_string(L) + ", " + std::to_string(p.f_xv < 0 ? -p.f_xv : p.f_xv) +
+        (p.f_xv < 0 ? ", true>" : ", false>");
   } else if (pass == 2) {
     d = std::string("kf_xconv<") + T + ", " + std::to_string(L) + "> (wz " + std::to_string(p.f_wz) + ")";
   } else if (pass == 1 || pass == 3) {

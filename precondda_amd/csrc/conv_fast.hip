@@ -365,15 +365,15 @@ bool xconv_aligned(const ConvArgs& a) {
   return ((uintptr_t)a.table & 15) == 0 && (a.table_stride & 1) == 0;
 }
 
-template <typename T, int L, int V> void launch_x(ConvPlanBase* p, const ConvArgs& a, hipStream_t s) {
+template <typename T, int L, int V, bool ST> void launch_x(ConvPlanBase* p, const ConvArgs& a, hipStream_t s) {
   if constexpr (XPlan<L>::ok) {
     const ConvGeom& g = p->g;
     constexpr int NT = XTile<L, V>::NT;
-    const size_t lds = (size_t)XTile<L, V>::lds_elems() * sizeof(C<T>);
+    const size_t lds = (size_t)XTile<L, V>::template lds_bytes<T>(ST);
     // workgroups resident at once on the device (persistent grid)
     static const int per_cu = [&] {
       int nb = 0;
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kx_xconv<T, L, V>, NT, lds) != hipSuccess || nb < 1)
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kx_xconv<T, L, V, ST>, NT, lds) != hipSuccess || nb < 1)
         nb = 1;
       return nb;
     }();
@@ -382,7 +382,7 @@ template <typename T, int L, int V> void launch_x(ConvPlanBase* p, const ConvArg
     long long nblk = std::min(ntiles, resident);
     nblk = std::max(8LL, (nblk + 7) / 8 * 8);
     const T scale = T(1.0 / ((double)g.Lx * g.Ly * g.Lz));
-    hipLaunchKernelGGL((kx_xconv<T, L, V>), dim3((unsigned)nblk), dim3(NT), lds, s, (C<T>*)p->work_b,
+    hipLaunchKernelGGL((kx_xconv<T, L, V, ST>), dim3((unsigned)nblk), dim3(NT), lds, s, (C<T>*)p->work_b,
                        (const C<T>*)a.table, a.table_stride, g, (const C<T>*)p->tw[0], a.ctl, a.need, scale);
   }
 }
@@ -390,12 +390,15 @@ template <typename T, int L, int V> void launch_x(ConvPlanBase* p, const ConvArg
 template <typename T, int L> void launch_p3(ConvPlanBase* p, const ConvArgs& a, hipStream_t s) {
   const ConvGeom& g = p->g;
   if (p->f_xv && xconv_aligned(a)) {
+    // f_xv: kz per tile; a negative value selects the LDS-staged table
     if constexpr (sizeof(T) == 4) {
-      if (p->f_xv == 8) return launch_x<T, L, 8>(p, a, s);
-      if (p->f_xv == 16) return launch_x<T, L, 16>(p, a, s);
+      if (p->f_xv == 8) return launch_x<T, L, 8, false>(p, a, s);
+      if (p->f_xv == 16) return launch_x<T, L, 16, false>(p, a, s);
+      if (p->f_xv == -8) return launch_x<T, L, 8, true>(p, a, s);
     } else {
-      if (p->f_xv == 4) return launch_x<T, L, 4>(p, a, s);
-      if (p->f_xv == 8) return launch_x<T, L, 8>(p, a, s);
+      if (p->f_xv == 4) return launch_x<T, L, 4, false>(p, a, s);
+      if (p->f_xv == 8) return launch_x<T, L, 8, false>(p, a, s);
+      if (p->f_xv == -4) return launch_x<T, L, 4, true>(p, a, s);
     }
   }
   const int nblk = g.nslots * (g.Ly / 2 + 1) * ((g.Lz + p->f_wz - 1) / p->f_wz);
@@ -484,13 +487,30 @@ std::vector<int> xconv_candidates(int L, int prec) {
   int P = 0, Q = 0;
   if (!xplan_pq(L, &P, &Q)) return out;
   const size_t es = prec ? 16 : 8;
-  const int vs[2] = {prec ? 4 : 8, prec ? 8 : 16};
+  // V > 0: table read from L2 in step B; V < 0: table slice staged in LDS (|V| kz)
+  const int vs[3] = {prec ? 4 : 8, prec ? 8 : 16, prec ? -4 : -8};
   for (int V : vs) {
-    const size_t elems = 2 * (size_t)L + (size_t)V * 3 * P * (Q + 1);
-    if (3 * P * V > 1024 || elems * es > device_lds_limit()) continue;
+    const int av = V < 0 ? -V : V;
+    size_t bytes = (2 * (size_t)L + (size_t)av * 3 * P * (Q + 1)) * es;
+    if (V < 0) {
+      const size_t nch = (size_t)(L / 2 + 1) * av * (6 * es / 16), nt = 3 * P * av;
+      bytes += (nch + nt - 1) / nt * nt * 16;
+    }
+    // the staged copy is issued by whole waves only
+    if (3 * P * av > 1024 || bytes > device_lds_limit() || (V < 0 && (3 * P * av) % 64 != 0)) continue;
     out.push_back(V);
   }
   return out;
+}
+
+int p3_forced_xv();
+// true when PDDA_P3_XV names a register variant this plan supports
+bool fxv_of_plan(const ConvPlanBase* p) {
+  const int fx = p3_forced_xv();
+  if (!fx || p->g.Lx != 2 * p->g.nx) return false;
+  for (int v : xconv_candidates(p->g.Lx, p->prec))
+    if (v == fx) return true;
+  return false;
 }
 
 // PDDA_P3_KERNEL=stockham|register forces the P3 kernel family (both are exact
@@ -498,6 +518,13 @@ std::vector<int> xconv_candidates(int L, int prec) {
 bool autotune_log() {
   const char* e = getenv("PDDA_AUTOTUNE_LOG");
   return e && e[0] == '1';
+}
+
+// PDDA_P3_XV=<v> forces one register-product variant (kz per tile; negative = staged
+// table) when the plan supports it; used to test each variant.
+int p3_forced_xv() {
+  const char* e = getenv("PDDA_P3_XV");
+  return e ? atoi(e) : 0;
 }
 
 int p3_forced() {
@@ -558,6 +585,9 @@ void fast_setup(ConvPlanBase* p) {
     const std::vector<int> xv = g.Lx == 2 * g.nx ? xconv_candidates(g.Lx, prec) : std::vector<int>();
     if (!xv.empty() && p3_forced() != 1) {
       p->f_xv = xv[0];
+      const int fx = p3_forced_xv();
+      for (int v : xv)
+        if (fx && v == fx) p->f_xv = v;
       p->fast[2] = 1;
     }
     if (p3_forced() == 2 && xv.empty()) p->f_xv = 0;
@@ -657,7 +687,7 @@ int fast_autotune(ConvPlanBase* p) {
     const int forced = p3_forced();
     float best = 1e30f;
     int bestw = p->f_wz, bestx = p->f_xv;
-    if (forced != 2 && p->f_wz > 0) {
+    if (forced != 2 && !fxv_of_plan(p) && p->f_wz > 0) {
       p->f_xv = 0;
       for (int w : pow2_candidates(g.Lx, p->prec, 6, pmax)) {
         p->f_wz = w;
@@ -667,8 +697,10 @@ int fast_autotune(ConvPlanBase* p) {
         if (t < best) { best = t; bestw = w; bestx = 0; }
       }
     }
+    const int fxv = p3_forced_xv();
     if (forced != 1 && g.Lx == 2 * g.nx) {
       for (int v : xconv_candidates(g.Lx, p->prec)) {
+        if (fxv && v != fxv) continue;
         p->f_xv = v;
         const float t = time_passes(p, passes, np, a, s, e0, e1, reps);
         if (autotune_log()) fprintf(stderr, "[pdda autotune] P3 register V=%d: %.2f us\n", v, 1e3 * t);
@@ -691,7 +723,7 @@ int fast_autotune(ConvPlanBase* p) {
 }
 
 bool fast_p3_usable(const ConvPlanBase* p, const ConvArgs& a) {
-  return p->fast[2] && (p->f_wz > 0 || (p->f_xv > 0 && xconv_aligned(a)));
+  return p->fast[2] && (p->f_wz > 0 || (p->f_xv != 0 && xconv_aligned(a)));
 }
 
 int fast_launch(ConvPlanBase* p, int pass, const ConvArgs& a, hipStream_t s) {

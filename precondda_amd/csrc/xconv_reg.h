@@ -80,7 +80,17 @@ template <int L, int V> struct XTile {
   static constexpr int RS = Q + 1;          // LDS row stride (odd: spreads banks)
   static constexpr int HX = L / 2;
   static constexpr int lds_elems() { return 2 * L + V * 3 * P * RS; }
+  // staged table: 16-byte chunks of the tile's (HX+1) x V entries, in rounds of NT
+  template <typename T> static constexpr int chunks_per_entry() { return 6 * (int)sizeof(C<T>) / 16; }
+  template <typename T> static constexpr int table_chunks() { return (HX + 1) * V * chunks_per_entry<T>(); }
+  template <typename T> static constexpr int table_rounds() { return (table_chunks<T>() + NT - 1) / NT; }
+  template <typename T> static constexpr int lds_bytes(bool staged) {
+    return lds_elems() * (int)sizeof(C<T>) + (staged ? table_rounds<T>() * NT * 16 : 0);
+  }
 };
+
+typedef __attribute__((address_space(3))) void lds_void_t;
+typedef __attribute__((address_space(1))) void glb_void_t;
 
 // The six reduced-table components of one (mx, my, mz) entry as 16-byte loads.
 template <typename T> __device__ inline void load_entry(const C<T>* __restrict__ p, C<T> (&g)[6]) {
@@ -107,7 +117,7 @@ template <typename T> __device__ inline void load_entry(const C<T>* __restrict__
 // contiguous ranges, one per XCD (one L2). Each workgroup prefetches the next tile's
 // inputs into registers while it transforms the current one. The table is read from
 // L2 in step B, six components per 16-byte-aligned entry.
-template <typename T, int L, int V>
+template <typename T, int L, int V, bool ST>
 __global__ __launch_bounds__(3 * XPlan<L>::P * V) void kx_xconv(C<T>* __restrict__ B, const C<T>* __restrict__ table,
                                                             long long table_stride, ConvGeom g,
                                                             const C<T>* __restrict__ twx, const SlotCtl* ctl,
@@ -118,6 +128,10 @@ __global__ __launch_bounds__(3 * XPlan<L>::P * V) void kx_xconv(C<T>* __restrict
   C<T>* twa = reinterpret_cast<C<T>*>(smem);  // scale * w^(n k1), step A
   C<T>* twb = twa + L;                        // w^(n k1), step B
   C<T>* dat = twb + L;
+  // staged table (ST): chunk (v, mx, part) at 16-byte slot (v * (HX + 1) + mx) * E16 + part
+  constexpr int E16 = X::template chunks_per_entry<T>();
+  constexpr int NCH = X::template table_chunks<T>(), NRND = X::template table_rounds<T>();
+  unsigned char* tabs = reinterpret_cast<unsigned char*>(dat + V * 3 * P * RS);
 
   const int hy = g.Ly / 2, hz = g.Lz / 2;
   const int nkzb = (g.Lz + V - 1) / V;
@@ -175,9 +189,32 @@ __global__ __launch_bounds__(3 * XPlan<L>::P * V) void kx_xconv(C<T>* __restrict
   while (lt < tend) {
     const int ln = lt + nper;
     const Tile nxt = decode(ln);
-    // orders the twiddle writes, and the previous tile's step C reads of `dat`, before
-    // this tile's step A writes
+    // orders the twiddle writes, and the previous tile's step C reads of `dat` (and step
+    // B reads of the staged table), before this tile's writes
     __syncthreads();
+    if constexpr (ST) {
+      if (cur.ok) {
+        // asynchronous global -> LDS copy of the tile's table slice; every lane of every
+        // wave takes part in each round (lanes past the slice copy chunk 0 into padding)
+        const char* tgb = reinterpret_cast<const char*>(table + cur.slot * table_stride +
+                                                        (long long)cur.qy * (hz + 1) * 6);
+        const int wave0 = t & ~63;
+#pragma unroll
+        for (int rr = 0; rr < NRND; ++rr) {
+          const int c = rr * NT + t;
+          long long boff = 0;
+          if (c < NCH) {
+            const int v = c / ((HX + 1) * E16), rem = c - v * (HX + 1) * E16;
+            const int mx = rem / E16, part = rem - mx * E16;
+            const int kz = min(cur.kz0 + v, g.Lz - 1);
+            const int mz = kz > hz ? g.Lz - kz : kz;
+            boff = ((long long)mx * mxs + mz * 6) * (long long)sizeof(C<T>) + part * 16;
+          }
+          __builtin_amdgcn_global_load_lds((glb_void_t*)(tgb + boff), (lds_void_t*)(tabs + (rr * NT + wave0) * 16),
+                                           16, 0, 0);
+        }
+      }
+    }
     if (cur.ok) {
       // step A: Q-point DFT with the upper half zero, times scale w^(n1 k1)
       C<T> z[Q];
@@ -188,6 +225,7 @@ __global__ __launch_bounds__(3 * XPlan<L>::P * V) void kx_xconv(C<T>* __restrict
 #pragma unroll
       for (int k1 = 0; k1 < Q; ++k1) row[k1] = cmul(z[k1], twa[n1a * Q + k1]);
     }
+    if constexpr (ST) __builtin_amdgcn_s_waitcnt(0);  // this wave's table chunks have landed
     // next tile's inputs in flight during steps B and C
     load_inputs(nxt, zc);
     __syncthreads();
@@ -213,7 +251,13 @@ __global__ __launch_bounds__(3 * XPlan<L>::P * V) void kx_xconv(C<T>* __restrict
           const bool fx = kx > HX;
           const int mx = fx ? L - kx : kx;
           C<T> ge[6];
-          load_entry<T>(tg + mx * mxs, ge);
+          if constexpr (ST) {
+            const C<T>* te = reinterpret_cast<const C<T>*>(tabs + (size_t)(v * (HX + 1) + mx) * E16 * 16);
+#pragma unroll
+            for (int i = 0; i < 6; ++i) ge[i] = te[i];
+          } else {
+            load_entry<T>(tg + mx * mxs, ge);
+          }
           C<T> h1 = ge[1], h2 = ge[2], h4 = ge[4];
           if (fx != fy) h1 = mk<T>(-h1.x, -h1.y);
           if (fx != fz) h2 = mk<T>(-h2.x, -h2.y);

@@ -87,3 +87,14 @@ def test_p3_unaligned_table_falls_back(monkeypatch, kernel):
     ref = O.interaction_mult(x, G.reshape(grid[0] + 1, grid[1] + 1, grid[2] + 1, 6), np.repeat(ia, 3), mask, grid)
     err = np.linalg.norm(out.cpu().numpy() - ref) / np.linalg.norm(ref)
     assert err < 2e-6
+
+
+@pytest.mark.parametrize("prec,xv", [(0, 8), (0, 16), (0, -8), (1, 4), (1, 8), (1, -4)])
+@pytest.mark.parametrize("grid", [(16, 16, 16), (32, 20, 6), (64, 6, 8), (128, 4, 4)])
+def test_p3_register_variants(monkeypatch, grid, prec, xv):
+    """Each register-product variant (kz per tile; negative = table staged in LDS by
+    direct global-to-LDS copies) where the plan supports it."""
+    monkeypatch.setenv("PDDA_P3_XV", str(xv))
+    err = _run(monkeypatch, "register", grid, prec, 3, False)
+    print("xv", xv, grid, prec, "rel err", err, flush=True)
+    assert err < (1e-12 if prec else 2e-6)
