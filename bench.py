@@ -244,8 +244,8 @@ def main():
         with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                                "matvec_64cube_traffic.json")) as f:
             tr = json.load(f)
-        kname = mv["c64"]["kernels"][dom]
-        if tuple(tr["grid"]) == tuple(wl.grid) and kname in tr["kernels"]:
+        kname = mv["c64"]["kernels"][dom].split(" (")[0]
+        if tuple(tr["grid"]) == tuple(wl.grid) and kname in tr["kernels"] and dom == 2:
             roofline["traffic"] = tr["kernels"][kname]["traffic_bytes"]
             roofline["traffic_unit"] = "bytes per launch (rocprofv3 FETCH_SIZE x%.0f + WRITE_SIZE)" % tr["fetch_factor"]
             roofline["algorithmic_bytes"] = int(byts64[dom])

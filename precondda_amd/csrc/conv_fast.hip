@@ -604,18 +604,23 @@ std::vector<int> pow2_candidates(int L, int prec, int c3, int pmax) {
   return out;
 }
 
-// Mean milliseconds of `reps` launches of the given passes.
+// Milliseconds per launch sequence of the given passes: the best of three trials of
+// `reps` launches each (candidates differ by a few percent; one trial is too noisy).
 float time_passes(ConvPlanBase* p, const int* passes, int np, const ConvArgs& a, hipStream_t s, hipEvent_t e0,
                   hipEvent_t e1, int reps) {
   for (int q = 0; q < np; ++q) fast_launch(p, passes[q], a, s);  // warm-up
-  (void)hipEventRecord(e0, s);
-  for (int r = 0; r < reps; ++r)
-    for (int q = 0; q < np; ++q) fast_launch(p, passes[q], a, s);
-  (void)hipEventRecord(e1, s);
-  (void)hipEventSynchronize(e1);
-  float ms = 0.f;
-  (void)hipEventElapsedTime(&ms, e0, e1);
-  return ms / reps;
+  float best = 1e30f;
+  for (int trial = 0; trial < 3; ++trial) {
+    (void)hipEventRecord(e0, s);
+    for (int r = 0; r < reps; ++r)
+      for (int q = 0; q < np; ++q) fast_launch(p, passes[q], a, s);
+    (void)hipEventRecord(e1, s);
+    (void)hipEventSynchronize(e1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    best = std::min(best, ms / reps);
+  }
+  return best;
 }
 
 }  // namespace
