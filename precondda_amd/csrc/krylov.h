@@ -44,12 +44,12 @@ template <typename T> struct SolverT {
   struct GraphEntry {
     hipGraphExec_t exec = nullptr;
     int n = 0;
-    hipStream_t stream = nullptr;
     bool replace = false;
     int sidx = 0;
   };
   static constexpr int kMaxGraphs = 8;
   std::vector<GraphEntry> graphs;
+  hipStream_t cap_stream = nullptr;  // private capture stream (default priority)
   // bound operands (device pointers, plan precision)
   const C<T>* ghat = nullptr;
   long long ghat_stride = 0;

@@ -335,6 +335,8 @@ void SolverT<T>::release() {
   for (GraphEntry& g : graphs)
     if (g.exec) (void)hipGraphExecDestroy(g.exec);
   graphs.clear();
+  if (cap_stream) (void)hipStreamDestroy(cap_stream);
+  cap_stream = nullptr;
   conv_plan_free(&mv);
   if (precon) conv_plan_free(&pc);
   C<T>* bufs[] = {vs.b, vs.r, vs.rhat, vs.p, vs.x, vs.best, vs.w, vs.u, vs.z, vs.s0, vs.s1,
@@ -457,8 +459,11 @@ int SolverT<T>::iterate_graph(int n_iter, hipStream_t s) {
   const bool rep = !lockstep;
   GraphEntry* ge = nullptr;
   for (GraphEntry& g : graphs)
-    if (g.n == n_iter && g.stream == s && g.replace == rep && g.sidx == sidx) ge = &g;
+    if (g.n == n_iter && g.replace == rep && g.sidx == sidx) ge = &g;
   if (!ge) {
+    // captured on a private stream (the caller's may be the legacy default stream,
+    // which cannot capture); the graph is then launched on the caller's stream
+    if (!cap_stream) PDDA_HIP_CHECK(hipStreamCreateWithFlags(&cap_stream, hipStreamNonBlocking));
     if ((int)graphs.size() >= kMaxGraphs) {  // drop the oldest
       (void)hipGraphExecDestroy(graphs.front().exec);
       graphs.erase(graphs.begin());
@@ -466,9 +471,9 @@ int SolverT<T>::iterate_graph(int n_iter, hipStream_t s) {
     hipGraph_t gr = nullptr;
     const long long hi = host_iter;
     const int si = sidx;
-    PDDA_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
-    int rc = iterate(n_iter, s);
-    hipError_t e = hipStreamEndCapture(s, &gr);
+    PDDA_HIP_CHECK(hipStreamBeginCapture(cap_stream, hipStreamCaptureModeRelaxed));
+    int rc = iterate(n_iter, cap_stream);
+    hipError_t e = hipStreamEndCapture(cap_stream, &gr);
     host_iter = hi;  // capture enqueued nothing
     sidx = si;
     if (rc) { if (gr) (void)hipGraphDestroy(gr); return rc; }
@@ -478,7 +483,6 @@ int SolverT<T>::iterate_graph(int n_iter, hipStream_t s) {
     (void)hipGraphDestroy(gr);
     if (e != hipSuccess) { set_error(std::string("graph instantiate: ") + hipGetErrorString(e)); return PDDA_ERR_HIP; }
     g.n = n_iter;
-    g.stream = s;
     g.replace = rep;
     g.sidx = si;
     graphs.push_back(g);
@@ -610,7 +614,9 @@ int MixedSolver::solve(const void* E64, double tol, double inner_target, int max
     }
     int it_done = 0;
     while (true) {
-      if ((rc = inner.iterate(check_every, s))) return rc;
+      // replayed from a captured graph when the ping-pong parity allows (even counts)
+      rc = (check_every % 2 == 0) ? inner.iterate_graph(check_every, s) : inner.iterate(check_every, s);
+      if (rc) return rc;
       it_done += check_every;
       if ((rc = inner.status(s))) return rc;
       bool running = false;

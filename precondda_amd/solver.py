@@ -79,11 +79,21 @@ class DeviceSolver:
         check(lib().pdda_solver_status(self._h, arr, self.nslots, stream_ptr(stream)))
         return list(arr)
 
-    def run(self, check_every: int = 8, max_iter: Optional[int] = None, stream=None) -> List[SlotStatus]:
-        """Iterates until every slot has stopped, reading the states every `check_every`."""
+    def iterate_graph(self, n: int, stream=None):
+        """Same iterations replayed from a captured hipGraph (n even)."""
+        check(lib().pdda_solver_iterate_graph(self._h, int(n), stream_ptr(stream)))
+
+    def run(self, check_every: int = 8, max_iter: Optional[int] = None, stream=None,
+            use_graph: bool = True) -> List[SlotStatus]:
+        """Iterates until every slot has stopped, reading the states every `check_every`
+        (graph replay when check_every is even and use_graph is set)."""
         done_iters = 0
+        graph = use_graph and check_every % 2 == 0
         while True:
-            self.iterate(check_every, stream)
+            if graph:
+                self.iterate_graph(check_every, stream)
+            else:
+                self.iterate(check_every, stream)
             done_iters += check_every
             st = self.status(stream)
             if all(s.status != _lib.ST_RUNNING for s in st):
