@@ -37,6 +37,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--inner-target", type=float, default=1e-3,
+                    help="relative residual each complex64 inner solve stops at (mixed path)")
     ap.add_argument("--profile-matvec", action="store_true",
                     help="only run the matvec kernels (complex64 and complex128, 20 each) and one 256 MiB "
                          "device copy, for rocprofv3 counter collection")
@@ -129,7 +131,8 @@ def main():
     ms.bind(G64, G32, M32, ia64, ia32, mask)
 
     def mixed_step():
-        ms.solve(E64, tol=args.tol, inner_target=1e-3, max_outer=12, max_inner=3000, ratio=100.0, check_every=8)
+        ms.solve(E64, tol=args.tol, inner_target=args.inner_target, max_outer=12, max_inner=3000, ratio=100.0,
+                 check_every=8)
         return ms.result()
 
     for w in range(args.warmup):

@@ -236,6 +236,8 @@ int pdda_conv_describe(const pdda_conv* c, int pass, char* buf, int n) {
   } else if (pass == 1 || pass == 3) {
     d = std::string("kf_axis<") + T + ", " + std::to_string(L) + ", " + (pass == 3 ? "true" : "false") + "> (cols " +
         std::to_string(p.f_ncol) + ")";
+  } else if ((pass == 0 && (p.f_zr & 1)) || (pass == 4 && (p.f_zr & 2))) {
+    d = std::string(pass == 0 ? "kz_zfwd<" : "kz_zinv<") + T + ", " + std::to_string(L) + ">";
   } else {
     d = std::string(pass == 0 ? "kf_zfwd<" : "kf_zinv<") + T + ", " + std::to_string(L) + "> (rows " +
         std::to_string(p.f_rows) + ")";

@@ -13,6 +13,7 @@
 #include "conv.h"
 #include "fft_ct.h"
 #include "xconv_reg.h"
+#include "zreg.h"
 
 namespace pdda {
 
@@ -326,6 +327,15 @@ int pick_pow2(int L, int prec, int c3, int want, int pmax) {
 
 template <typename T, int L> void launch_p1(ConvPlanBase* p, const ConvArgs& a, hipStream_t s) {
   const ConvGeom& g = p->g;
+  if constexpr (XPlan<L>::ok) {
+    if (p->f_zr & 1) {
+      const long long tiles = ((long long)g.nx * g.ny + ZTile<L>::R - 1) / ZTile<L>::R;
+      hipLaunchKernelGGL((kz_zfwd<T, L>), dim3((unsigned)(g.nslots * tiles)), dim3(ZTile<L>::NT),
+                         (size_t)ZTile<L>::lds_elems() * sizeof(C<T>), s, (const C<T>*)a.x, (C<T>*)p->work_a, g,
+                         (const C<T>*)p->tw[2], a.ctl, a.need);
+      return;
+    }
+  }
   const int nblk = g.nslots * g.nx * ((g.ny + p->f_rows - 1) / p->f_rows);
   hipLaunchKernelGGL((kf_zfwd<T, L>), dim3(nblk), dim3(FFT_NT), p->f_lds[0], s, (const C<T>*)a.x,
                      (C<T>*)p->work_a, g, (const C<T>*)p->tw[2], a.ctl, a.need, p->f_rows);
@@ -409,6 +419,16 @@ template <typename T, int L> void launch_p3(ConvPlanBase* p, const ConvArgs& a, 
 
 template <typename T, int L, int DOTS> void launch_p5d(ConvPlanBase* p, const ConvArgs& a, hipStream_t s) {
   const ConvGeom& g = p->g;
+  if constexpr (XPlan<L>::ok) {
+    if (p->f_zr & 2) {
+      hipLaunchKernelGGL((kz_zinv<T, L, DOTS>), dim3((unsigned)(g.nslots * p->nwg5)), dim3(ZTile<L>::NT),
+                         (size_t)ZTile<L>::lds_elems() * sizeof(C<T>), s, (const C<T>*)p->work_a, (C<T>*)a.out, g,
+                         (const C<T>*)p->tw[2], a.mode, (const C<T>*)a.x, (const C<T>*)a.inv_alpha, a.mask,
+                         a.site_stride, (const C<T>*)a.b, (const C<T>*)a.d0, (const C<T>*)a.d1, p->partial, a.ctl,
+                         a.need, p->nwg5);
+      return;
+    }
+  }
   const int nblk = g.nslots * p->nwg5;
   hipLaunchKernelGGL((kf_zinv<T, L, DOTS>), dim3(nblk), dim3(FFT_NT), p->f_lds[4], s, (const C<T>*)p->work_a,
                      (C<T>*)a.out, g, (const C<T>*)p->tw[2], a.mode, (const C<T>*)a.x, (const C<T>*)a.inv_alpha,
@@ -465,6 +485,28 @@ bool xplan_pq(int L, int* P, int* Q) {
     case 192: *P = 12; *Q = 16; return true;
     case 256: *P = 16; *Q = 16; return true;
     default: return false;
+  }
+}
+
+// Rows per block of the register z kernels (192 threads / (3 max(P, Q))); 0 if none.
+int zreg_rows(int L) {
+  int P = 0, Q = 0;
+  if (!xplan_pq(L, &P, &Q)) return 0;
+  return 192 / (3 * (P > Q ? P : Q));
+}
+
+// zr: bit 0 = register P1, bit 1 = register P5; rows = LDS row tile of the others.
+void set_z_tiles(ConvPlanBase* p, int zr, int rows) {
+  const ConvGeom& g = p->g;
+  const size_t es = p->prec ? 16 : 8;
+  p->f_zr = zr;
+  p->f_rows = rows;
+  p->f_lds[0] = p->f_lds[4] = (size_t)(g.Lz + (size_t)g.Lz * 3 * rows) * es;
+  if (zr & 2) {
+    const int R = zreg_rows(g.Lz);
+    p->nwg5 = (int)(((long long)g.nx * g.ny + R - 1) / R);
+  } else {
+    p->nwg5 = g.nx * ((g.ny + rows - 1) / rows);
   }
 }
 
@@ -525,6 +567,15 @@ bool autotune_log() {
 int p3_forced_xv() {
   const char* e = getenv("PDDA_P3_XV");
   return e ? atoi(e) : 0;
+}
+
+// PDDA_Z_KERNEL=stockham|register forces the P1/P5 kernel family.
+int z_forced() {
+  const char* e = getenv("PDDA_Z_KERNEL");
+  if (!e) return 0;
+  if (e[0] == 's') return 1;
+  if (e[0] == 'r') return 2;
+  return 0;
 }
 
 int p3_forced() {
@@ -592,7 +643,7 @@ void fast_setup(ConvPlanBase* p) {
     }
     if (p3_forced() == 2 && xv.empty()) p->f_xv = 0;
   }
-  if (p->fast[0]) p->nwg5 = g.nx * ((g.ny + p->f_rows - 1) / p->f_rows);
+  if (p->fast[0]) set_z_tiles(p, (z_forced() == 2 && zreg_rows(g.Lz) > 0) ? 3 : 0, p->f_rows);
 }
 
 namespace {
@@ -651,19 +702,34 @@ int fast_autotune(ConvPlanBase* p) {
   if (p->fast[0]) {
     int pmax = 1;
     while (pmax * 2 <= g.ny) pmax *= 2;
-    const int passes[2] = {0, 4};
+    // timed inside the whole P1..P5 sequence (alone, P5 can look faster than it runs in
+    // place after P4)
+    int passes[5], np = 0;
+    for (int q = 0; q < 5; ++q)
+      if (p->fast[q]) passes[np++] = q;
     float best = 1e30f;
-    int bestr = p->f_rows;
-    for (int r : pow2_candidates(g.Lz, p->prec, 3, pmax)) {
-      p->f_rows = r;
-      p->f_lds[0] = p->f_lds[4] = (size_t)(g.Lz + (size_t)g.Lz * 3 * r) * es;
-      p->nwg5 = g.nx * ((g.ny + r - 1) / r);
-      const float t = time_passes(p, passes, 2, a, s, e0, e1, reps);
-      if (t < best) { best = t; bestr = r; }
+    int bestr = p->f_rows, bestz = 0;
+    const int zforce = z_forced();
+    if (zforce != 2) {
+      for (int r : pow2_candidates(g.Lz, p->prec, 3, pmax)) {
+        set_z_tiles(p, 0, r);
+        const float t = time_passes(p, passes, np, a, s, e0, e1, reps);
+        if (autotune_log()) fprintf(stderr, "[pdda autotune] P1+P5 rows=%d: %.2f us\n", r, 1e3 * t);
+        if (t < best) { best = t; bestr = r; bestz = 0; }
+      }
     }
-    p->f_rows = bestr;
-    p->f_lds[0] = p->f_lds[4] = (size_t)(g.Lz + (size_t)g.Lz * 3 * bestr) * es;
-    p->nwg5 = g.nx * ((g.ny + bestr - 1) / bestr);
+    if (zforce != 1 && zreg_rows(g.Lz) > 0) {
+      // register P1 beside the best LDS row tile. The register P5 runs its epilogue on
+      // half the block; with the solver's fused dot products it ran 5% slower per
+      // solve than the LDS P5, so it is used only when forced (PDDA_Z_KERNEL=register).
+      for (int zr = (zforce == 2 ? 3 : 1); zr <= (zforce == 2 ? 3 : 1); ++zr) {
+        set_z_tiles(p, zr, bestr);
+        const float t = time_passes(p, passes, np, a, s, e0, e1, reps);
+        if (autotune_log()) fprintf(stderr, "[pdda autotune] P1+P5 register mask %d: %.2f us\n", zr, 1e3 * t);
+        if (t < best || zforce == 2) { best = t; bestz = zr; }
+      }
+    }
+    set_z_tiles(p, bestz, bestr);
   }
   if (p->fast[1]) {
     int pmax = 1;

@@ -1,0 +1,185 @@
+// Passes P1 (z forward) and P5 (z inverse + epilogue) as four-step FFTs over whole
+// box rows, with one LDS exchange per transform and global I/O straight from registers.
+//
+// A row is (slot, ix, iy): nz sites x 3 components, contiguous (component fastest).
+// With Lz = P Q, n = n1 + P n2, k = k1 + Q k2 (the factorisation of xconv_reg.h):
+//   P1  step A (thread per component, n1): loads n1 + P n2 < nz, Q-point DFT of the
+//       non-zero half, times w^(n1 k1), LDS row [n1][k1];
+//       step B (thread per component, k1): P-point DFT over n1 -> A[k1 + Q k2],
+//       stored straight to the row of A (lanes: component fastest, so a store covers
+//       consecutive elements).
+//   P5  the mirror image: inverse P-point DFTs from A into LDS, inverse Q-point DFTs
+//       over k1 for the kept positions n1 + P n2 < nz, then the epilogue (mask,
+//       alpha^-1 x - ..., b - ...) and the Krylov dot products on those registers.
+// Blocks are 192 threads (whole waves) covering R = 192 / (3 max(P, Q)) rows.
+#pragma once
+
+#include "xconv_reg.h"
+
+namespace pdda {
+
+template <int L> struct ZTile {
+  static constexpr int P = XPlan<L>::P, Q = XPlan<L>::Q;
+  static constexpr int NT = 192;
+  static constexpr int R = NT / (3 * (P > Q ? P : Q));  // rows per block
+  static constexpr int RS = Q + 1;
+  static constexpr int lds_elems() { return L + R * 3 * P * RS; }
+  static_assert(R >= 1 && NT % (3 * (P > Q ? P : Q)) == 0, "rows per block");
+};
+
+template <typename T, int L>
+__global__ __launch_bounds__(192) void kz_zfwd(const C<T>* __restrict__ x, C<T>* __restrict__ A, ConvGeom g,
+                                               const C<T>* __restrict__ twz, const SlotCtl* ctl, int need) {
+  using Z = ZTile<L>;
+  constexpr int P = Z::P, Q = Z::Q, R = Z::R, RS = Z::RS, NT = Z::NT;
+  extern __shared__ __align__(16) unsigned char smem[];
+  C<T>* tw = reinterpret_cast<C<T>*>(smem);  // tw[n * Q + k1] = w^(n k1)
+  C<T>* dat = tw + L;                        // rows (r, c, n1) x RS
+  const long long rows = (long long)g.nx * g.ny;
+  const long long tiles = (rows + R - 1) / R;
+  const int slot = (int)(blockIdx.x / tiles);
+  const long long row0 = (blockIdx.x - (long long)slot * tiles) * R;
+  if (!slot_wanted(ctl, slot, need)) return;
+  const int nz = g.nz, t = threadIdx.x;
+  for (int i = t; i < L; i += NT) tw[i] = twz[(i / Q) * (i % Q)];
+  // step A: items (r, n1, c), component fastest
+  C<T> z[Q];
+  const bool ia = t < 3 * P * R;
+  const int ca = t % 3, n1a = (t / 3) % P, ra = t / (3 * P);
+  const bool rowa = ia && row0 + ra < rows;
+  const C<T>* src = x + slot * g.vec_stride + (row0 + ra) * 3LL * nz + ca;
+#pragma unroll
+  for (int n2 = 0; n2 < Q / 2; ++n2) {
+    const int pos = n1a + P * n2;
+    z[n2] = (rowa && pos < nz) ? src[pos * 3] : mk<T>(T(0), T(0));
+  }
+  __syncthreads();  // twiddles
+  if (ia) {
+    dft_lowhalf<T, Q>(z);
+    C<T>* row = dat + ((ra * 3 + ca) * P + n1a) * RS;
+    row[0] = z[0];
+#pragma unroll
+    for (int k1 = 1; k1 < Q; ++k1) row[k1] = cmul(z[k1], tw[n1a * Q + k1]);
+  }
+  __syncthreads();
+  // step B: items (r, k1, c), component fastest
+  if (t < 3 * Q * R) {
+    const int cb = t % 3, k1 = (t / 3) % Q, rb = t / (3 * Q);
+    if (row0 + rb < rows) {
+      C<T> v[P];
+      const C<T>* col = dat + (rb * 3 + cb) * P * RS + k1;
+#pragma unroll
+      for (int n = 0; n < P; ++n) v[n] = col[n * RS];
+      DftC<T, P>::run(v);
+      C<T>* dst = A + slot * g.a_stride + (row0 + rb) * 3LL * L + cb;
+#pragma unroll
+      for (int k2 = 0; k2 < P; ++k2) dst[(k1 + Q * k2) * 3] = v[k2];
+    }
+  }
+}
+
+template <typename T, int L, int DOTS>
+__global__ __launch_bounds__(192) void kz_zinv(const C<T>* __restrict__ A, C<T>* __restrict__ out, ConvGeom g,
+                                               const C<T>* __restrict__ twz, int mode, const C<T>* __restrict__ xin,
+                                               const C<T>* __restrict__ ia, const uint8_t* __restrict__ mask,
+                                               long long site_stride, const C<T>* __restrict__ bvec,
+                                               const C<T>* __restrict__ d0, const C<T>* __restrict__ d1, cd* partial,
+                                               const SlotCtl* ctl, int need, int nwg) {
+  using Z = ZTile<L>;
+  constexpr int P = Z::P, Q = Z::Q, R = Z::R, RS = Z::RS, NT = Z::NT;
+  extern __shared__ __align__(16) unsigned char smem[];
+  C<T>* tw = reinterpret_cast<C<T>*>(smem);
+  C<T>* dat = tw + L;
+  const long long rows = (long long)g.nx * g.ny;
+  const long long tiles = (rows + R - 1) / R;
+  const int slot = (int)(blockIdx.x / tiles);
+  const int wg = (int)(blockIdx.x - (long long)slot * tiles);
+  const long long row0 = (long long)wg * R;
+  if (!slot_wanted(ctl, slot, need)) return;
+  const int nz = g.nz, t = threadIdx.x;
+  for (int i = t; i < L; i += NT) tw[i] = twz[(i / Q) * (i % Q)];
+  // step A': items (r, k1, c): inverse P-point DFT over k2 of A[k1 + Q k2]
+  C<T> v[P];
+  const bool ib = t < 3 * Q * R;
+  const int cb = t % 3, k1b = (t / 3) % Q, rb = t / (3 * Q);
+  const bool rowb = ib && row0 + rb < rows;
+  {
+    const C<T>* src = A + slot * g.a_stride + (row0 + rb) * 3LL * L + cb;
+#pragma unroll
+    for (int k2 = 0; k2 < P; ++k2) v[k2] = rowb ? cconj(src[(k1b + Q * k2) * 3]) : mk<T>(T(0), T(0));
+  }
+  __syncthreads();  // twiddles
+  if (ib) {
+    DftC<T, P>::run(v);
+    // conj(result) * conj(w^(n1 k1)) = conj(result * w^(n1 k1)): rows keep conjugates
+    C<T>* col = dat + (rb * 3 + cb) * P * RS + k1b;
+#pragma unroll
+    for (int n = 0; n < P; ++n) col[n * RS] = n == 0 ? v[0] : cmul(v[n], tw[n * Q + k1b]);
+  }
+  __syncthreads();
+  // step C': items (r, n1, c): inverse Q-point DFT over k1, kept positions n1 + P n2 < nz
+  constexpr int ND = DOTS == DOT_ALPHA ? 1 : DOTS == DOT_ZETA ? 6 : DOTS == DOT_RESID ? 2 : 1;
+  double sre[ND], sim[ND];
+#pragma unroll
+  for (int d = 0; d < ND; ++d) { sre[d] = 0.0; sim[d] = 0.0; }
+  const int ca = t % 3, n1a = (t / 3) % P, ra = t / (3 * P);
+  if (t < 3 * P * R && row0 + ra < rows) {
+    C<T> z[Q];
+    const C<T>* rowp = dat + ((ra * 3 + ca) * P + n1a) * RS;
+#pragma unroll
+    for (int k1 = 0; k1 < Q; ++k1) z[k1] = rowp[k1];
+    DftC<T, Q>::run(z);
+    const long long site0 = (row0 + ra) * nz;
+    const long long vbase = slot * g.vec_stride + site0 * 3 + ca;
+    const long long sbase = slot * site_stride + site0;
+#pragma unroll
+    for (int n2 = 0; n2 < Q / 2; ++n2) {
+      const int pos = n1a + P * n2;
+      if (pos >= nz) continue;
+      const C<T> cv = cconj(z[n2]);
+      const long long vidx = vbase + pos * 3;
+      C<T> o;
+      if (mode == MODE_FIELD) {
+        o = cv;
+      } else {
+        const C<T> mc = cscale(cv, mask[sbase + pos] ? T(1) : T(0));
+        if (mode == MODE_PRECOND) {
+          o = mc;
+        } else {
+          const C<T> w = csub(cmul(ia[sbase + pos], xin[vidx]), mc);
+          o = (mode == MODE_INTERACT) ? w : csub(bvec[vidx], w);
+        }
+      }
+      out[vidx] = o;
+      if (DOTS == DOT_ALPHA) {
+        const C<T> a = d0[vidx];
+        sre[0] += (double)a.x * o.x + (double)a.y * o.y;
+        sim[0] += (double)a.x * o.y - (double)a.y * o.x;
+      } else if (DOTS == DOT_RESID) {
+        const C<T> a = d0[vidx];
+        sre[0] += (double)o.x * o.x + (double)o.y * o.y;
+        sre[1] += (double)a.x * o.x + (double)a.y * o.y;
+        sim[1] += (double)a.x * o.y - (double)a.y * o.x;
+      } else if (DOTS == DOT_ZETA) {
+        const C<T> y = d0[vidx], s = d1[vidx];
+        const double yx = y.x, yy = y.y, tx = o.x, ty = o.y, sx = s.x, sy = s.y;
+        sre[0] += yx * yx + yy * yy;
+        sre[1] += tx * tx + ty * ty;
+        sre[2] += yx * tx + yy * ty; sim[2] += yx * ty - yy * tx;
+        sre[3] += tx * yx + ty * yy; sim[3] += tx * yy - ty * yx;
+        sre[4] += yx * sx + yy * sy; sim[4] += yx * sy - yy * sx;
+        sre[5] += tx * sx + ty * sy; sim[5] += tx * sy - ty * sx;
+      }
+    }
+  }
+  if (DOTS != DOT_NONE) {
+    __shared__ double scratch[2 * ND * (NT / 64)];
+    block_sum<ND>(sre, sim, scratch);
+    if (threadIdx.x == 0) {
+      cd* pp = partial + (size_t)slot * nwg * 6 + (size_t)wg * ND;
+      for (int d = 0; d < ND; ++d) pp[d] = cd{sre[d], sim[d]};
+    }
+  }
+}
+
+}  // namespace pdda

@@ -17,10 +17,12 @@ GRIDS = [(8, 6, 5), (12, 10, 8), (16, 16, 16), (24, 8, 12), (32, 20, 6), (48, 12
          (128, 4, 4)]
 
 
-def _run(monkeypatch, kernel, grid, prec, nslots, shared):
+def _run(monkeypatch, kernel, grid, prec, nslots, shared, zkernel=None):
     from precondda_amd import _lib
     from precondda_amd.operators import ConvPlan
     monkeypatch.setenv("PDDA_P3_KERNEL", kernel)
+    if zkernel:
+        monkeypatch.setenv("PDDA_Z_KERNEL", zkernel)
     nx, ny, nz = grid
     n = nx * ny * nz
     rng = np.random.default_rng(nx * 1000 + ny * 10 + nz)
@@ -97,4 +99,16 @@ def test_p3_register_variants(monkeypatch, grid, prec, xv):
     monkeypatch.setenv("PDDA_P3_XV", str(xv))
     err = _run(monkeypatch, "register", grid, prec, 3, False)
     print("xv", xv, grid, prec, "rel err", err, flush=True)
+    assert err < (1e-12 if prec else 2e-6)
+
+
+@pytest.mark.parametrize("prec", [1, 0])
+@pytest.mark.parametrize("grid", [(8, 6, 5), (12, 10, 8), (16, 16, 16), (24, 8, 12), (32, 20, 6), (64, 6, 8),
+                                  (8, 4, 48), (6, 4, 64), (4, 4, 96)])
+@pytest.mark.parametrize("zkernel", ["register", "stockham"])
+def test_z_passes_match_oracle(monkeypatch, zkernel, grid, prec):
+    """Both z-pass families (LDS row tiles, register rows) for z lengths 10..192, odd nz,
+    three slots with their own tables."""
+    err = _run(monkeypatch, "stockham", grid, prec, 3, False, zkernel=zkernel)
+    print(zkernel, grid, prec, "rel err", err, flush=True)
     assert err < (1e-12 if prec else 2e-6)
