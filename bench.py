@@ -273,17 +273,36 @@ def main():
             if args.precon:
                 M_np, _ = O.build_preconditioner(wl.grid, wl.k, 1.0, ia_np, P.operators.DEFAULT_REFRACT_MULT)
             E_np = wl.E[0].cpu().numpy()
-            n_it = 3
-            log("cpu baseline: timing")
-            t = time.perf_counter()
-            O.solve_gpbicgstab(wl.grid, ia_np, G_np, M_np, E_np, m_np, 100, n_it, args.tol, precon=args.precon)
-            dt_cpu = time.perf_counter() - t
-            per_it = dt_cpu / n_it
+            try:
+                model = next(l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name"))
+            except (OSError, StopIteration):
+                model = "unknown CPU"
             iters = max(abs(i) for i in plain_iters)
-            cpu = {"value": round(1.0 / (per_it * iters), 6), "unit": "solves/s", "cores": cores, "kind": "port",
-                   "sample": f"{n_it} iterations of the complex128 oracle GPBiCGSTAB (scipy.fft, {cores} threads) on "
-                             f"one RHS took {dt_cpu:.2f}s; extrapolated to the {iters} iterations the FP64 solve "
-                             f"needs on this workload"}
+            log("cpu baseline: timing")
+            # a 2-iteration probe sizes the sample: the whole solve of one RHS when it fits
+            # in ~30 s of host time, otherwise ~20 s of iterations, extrapolated
+            t = time.perf_counter()
+            O.solve_gpbicgstab(wl.grid, ia_np, G_np, M_np, E_np, m_np, 100, 2, args.tol, precon=args.precon)
+            per_it = (time.perf_counter() - t) / 2
+            if per_it * iters <= 30.0:
+                t = time.perf_counter()
+                res = O.solve_gpbicgstab(wl.grid, ia_np, G_np, M_np, E_np, m_np, 100, iters + 20, args.tol,
+                                         precon=args.precon)
+                dt_cpu = time.perf_counter() - t
+                cpu = {"value": round(1.0 / dt_cpu, 6), "unit": "solves/s", "cores": cores, "kind": "port",
+                       "sample": f"one whole RHS solve ({res[0]} iterations to relative residual {args.tol:g}) "
+                                 f"of the complex128 oracle GPBiCGSTAB (NumPy + scipy.fft, {cores} threads, {model}) "
+                                 f"took {dt_cpu:.2f}s"}
+            else:
+                n_it = max(2, int(20.0 / per_it))
+                t = time.perf_counter()
+                O.solve_gpbicgstab(wl.grid, ia_np, G_np, M_np, E_np, m_np, 100, n_it, args.tol, precon=args.precon)
+                dt_cpu = time.perf_counter() - t
+                cpu = {"value": round(n_it / (dt_cpu * iters), 6), "unit": "solves/s", "cores": cores,
+                       "kind": "port",
+                       "sample": f"{n_it} iterations of the complex128 oracle GPBiCGSTAB (NumPy + scipy.fft, {cores} "
+                                 f"threads, {model}) on one RHS took {dt_cpu:.2f}s; extrapolated to the {iters} "
+                                 f"iterations the FP64 solve needs on this workload"}
             log(f"cpu baseline {cpu}")
         except Exception as ex:  # the CPU leg is informational; report why it is absent
             cpu = {"value": None, "unit": "solves/s", "cores": 0, "kind": "port", "sample": f"failed: {ex}"}
