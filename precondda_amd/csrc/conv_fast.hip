@@ -393,7 +393,8 @@ template <typename T, int L, int V, bool ST> void launch_x(ConvPlanBase* p, cons
     nblk = std::max(8LL, (nblk + 7) / 8 * 8);
     const T scale = T(1.0 / ((double)g.Lx * g.Ly * g.Lz));
     hipLaunchKernelGGL((kx_xconv<T, L, V, ST>), dim3((unsigned)nblk), dim3(NT), lds, s, (C<T>*)p->work_b,
-                       (const C<T>*)a.table, a.table_stride, g, (const C<T>*)p->tw[0], a.ctl, a.need, scale);
+                       (const C<T>*)a.table, a.table_stride, g, (const C<T>*)p->tw[0], a.ctl, a.need, scale,
+                       p->f_xorder);
   }
 }
 
@@ -421,8 +422,9 @@ template <typename T, int L, int DOTS> void launch_p5d(ConvPlanBase* p, const Co
   const ConvGeom& g = p->g;
   if constexpr (XPlan<L>::ok) {
     if (p->f_zr & 2) {
-      hipLaunchKernelGGL((kz_zinv<T, L, DOTS>), dim3((unsigned)(g.nslots * p->nwg5)), dim3(ZTile<L>::NT),
-                         (size_t)ZTile<L>::lds_elems() * sizeof(C<T>), s, (const C<T>*)p->work_a, (C<T>*)a.out, g,
+      using ZI = ZTile<L, true>;
+      hipLaunchKernelGGL((kz_zinv<T, L, DOTS>), dim3((unsigned)(g.nslots * p->nwg5)), dim3(ZI::NT),
+                         (size_t)ZI::lds_elems() * sizeof(C<T>), s, (const C<T>*)p->work_a, (C<T>*)a.out, g,
                          (const C<T>*)p->tw[2], a.mode, (const C<T>*)a.x, (const C<T>*)a.inv_alpha, a.mask,
                          a.site_stride, (const C<T>*)a.b, (const C<T>*)a.d0, (const C<T>*)a.d1, p->partial, a.ctl,
                          a.need, p->nwg5);
@@ -494,6 +496,8 @@ int zreg_rows(int L) {
   if (!xplan_pq(L, &P, &Q)) return 0;
   return 192 / (3 * (P > Q ? P : Q));
 }
+// The same for the z inverse (its own factorisation; the larger factor is the same).
+int zreg_rows_inv(int L) { return zreg_rows(L); }
 
 // zr: bit 0 = register P1, bit 1 = register P5; rows = LDS row tile of the others.
 void set_z_tiles(ConvPlanBase* p, int zr, int rows) {
@@ -503,7 +507,7 @@ void set_z_tiles(ConvPlanBase* p, int zr, int rows) {
   p->f_rows = rows;
   p->f_lds[0] = p->f_lds[4] = (size_t)(g.Lz + (size_t)g.Lz * 3 * rows) * es;
   if (zr & 2) {
-    const int R = zreg_rows(g.Lz);
+    const int R = zreg_rows_inv(g.Lz);
     p->nwg5 = (int)(((long long)g.nx * g.ny + R - 1) / R);
   } else {
     p->nwg5 = g.nx * ((g.ny + rows - 1) / rows);
@@ -632,6 +636,7 @@ void fast_setup(ConvPlanBase* p) {
       p->fast[2] = 1;
     }
     p->f_xv = 0;
+    if (const char* e = getenv("PDDA_P3_ORDER")) p->f_xorder = atoi(e);
     // the register-product kernel reads and writes x rows n < Lx / 2, so it needs Lx = 2 nx
     const std::vector<int> xv = g.Lx == 2 * g.nx ? xconv_candidates(g.Lx, prec) : std::vector<int>();
     if (!xv.empty() && p3_forced() != 1) {
@@ -710,21 +715,28 @@ int fast_autotune(ConvPlanBase* p) {
     float best = 1e30f;
     int bestr = p->f_rows, bestz = 0;
     const int zforce = z_forced();
+    // P5 as the solver runs it: interaction mode with the six zeta/eta dot products
+    // (timing only: the operands alias the plan's workspaces)
+    ConvArgs az = a;
+    az.mode = MODE_INTERACT;
+    az.dots = DOT_ZETA;
+    az.inv_alpha = p->work_a;
+    az.mask = (const uint8_t*)p->work_b;
+    az.d0 = p->work_b;
+    az.d1 = p->work_b;
     if (zforce != 2) {
       for (int r : pow2_candidates(g.Lz, p->prec, 3, pmax)) {
         set_z_tiles(p, 0, r);
-        const float t = time_passes(p, passes, np, a, s, e0, e1, reps);
+        const float t = time_passes(p, passes, np, az, s, e0, e1, reps);
         if (autotune_log()) fprintf(stderr, "[pdda autotune] P1+P5 rows=%d: %.2f us\n", r, 1e3 * t);
         if (t < best) { best = t; bestr = r; bestz = 0; }
       }
     }
     if (zforce != 1 && zreg_rows(g.Lz) > 0) {
-      // register P1 beside the best LDS row tile. The register P5 runs its epilogue on
-      // half the block; with the solver's fused dot products it ran 5% slower per
-      // solve than the LDS P5, so it is used only when forced (PDDA_Z_KERNEL=register).
-      for (int zr = (zforce == 2 ? 3 : 1); zr <= (zforce == 2 ? 3 : 1); ++zr) {
+      // register P1, P5 or both, beside the best LDS row tile
+      for (int zr = (zforce == 2 ? 3 : 1); zr <= 3; ++zr) {
         set_z_tiles(p, zr, bestr);
-        const float t = time_passes(p, passes, np, a, s, e0, e1, reps);
+        const float t = time_passes(p, passes, np, az, s, e0, e1, reps);
         if (autotune_log()) fprintf(stderr, "[pdda autotune] P1+P5 register mask %d: %.2f us\n", zr, 1e3 * t);
         if (t < best || zforce == 2) { best = t; bestz = zr; }
       }

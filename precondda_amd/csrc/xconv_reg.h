@@ -121,7 +121,7 @@ template <typename T, int L, int V, bool ST>
 __global__ __launch_bounds__(3 * XPlan<L>::P * V) void kx_xconv(C<T>* __restrict__ B, const C<T>* __restrict__ table,
                                                             long long table_stride, ConvGeom g,
                                                             const C<T>* __restrict__ twx, const SlotCtl* ctl,
-                                                            int need, T scale) {
+                                                            int need, T scale, int order) {
   using X = XTile<L, V>;
   constexpr int P = X::P, Q = X::Q, NT = X::NT, RS = X::RS, HX = X::HX;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -154,12 +154,22 @@ __global__ __launch_bounds__(3 * XPlan<L>::P * V) void kx_xconv(C<T>* __restrict
     Tile u;
     u.ok = lt < tend;
     if (!u.ok) return u;
-    u.slot = lt % g.nslots;
-    int r = lt / g.nslots;
-    const int yr = r & 1;
-    r >>= 1;
-    const int kbp = r % nkzb;
-    u.qy = r / nkzb;
+    int yr, kbp;
+    if (order == 0) {  // slot fastest, then ky / mirror, then kz block, then ky pair
+      u.slot = lt % g.nslots;
+      int r = lt / g.nslots;
+      yr = r & 1;
+      r >>= 1;
+      kbp = r % nkzb;
+      u.qy = r / nkzb;
+    } else {  // kz block fastest, then ky / mirror, then slot, then ky pair
+      kbp = lt % nkzb;
+      int r = lt / nkzb;
+      yr = r & 1;
+      r >>= 1;
+      u.slot = r % g.nslots;
+      u.qy = r / g.nslots;
+    }
     u.kz0 = ((kbp & 1) ? nkzb - 1 - (kbp >> 1) : (kbp >> 1)) * V;
     u.ky = yr ? g.Ly - u.qy : u.qy;
     u.ok = !(yr && (u.qy == 0 || u.qy == hy)) && slot_wanted(ctl, u.slot, need);

@@ -11,15 +11,24 @@
 //   P5  the mirror image: inverse P-point DFTs from A into LDS, inverse Q-point DFTs
 //       over k1 for the kept positions n1 + P n2 < nz, then the epilogue (mask,
 //       alpha^-1 x - ..., b - ...) and the Krylov dot products on those registers.
-// Blocks are 192 threads (whole waves) covering R = 192 / (3 max(P, Q)) rows.
+// Blocks are 192 threads (whole waves) covering R = 192 / (3 max(P, Q)) rows. The
+// inverse uses the factorisation with P >= Q, so that its epilogue step fills the block.
 #pragma once
 
 #include "xconv_reg.h"
 
 namespace pdda {
 
-template <int L> struct ZTile {
-  static constexpr int P = XPlan<L>::P, Q = XPlan<L>::Q;
+// Factorisations of the z inverse with P >= Q (Q even): its last step, which carries
+// the epilogue, then has 3 P items per row and fills the block.
+template <int L> struct ZInvPlan { static constexpr int P = XPlan<L>::Q, Q = XPlan<L>::P; };
+template <> struct ZInvPlan<24> { static constexpr int P = 3, Q = 8; };  // 192 = 8 * 24 rows
+template <> struct ZInvPlan<256> { static constexpr int P = 16, Q = 16; };
+template <> struct ZInvPlan<64> { static constexpr int P = 8, Q = 8; };
+template <> struct ZInvPlan<16> { static constexpr int P = 4, Q = 4; };
+
+template <int L, bool INV = false> struct ZTile {
+  static constexpr int P = INV ? ZInvPlan<L>::P : XPlan<L>::P, Q = INV ? ZInvPlan<L>::Q : XPlan<L>::Q;
   static constexpr int NT = 192;
   static constexpr int R = NT / (3 * (P > Q ? P : Q));  // rows per block
   static constexpr int RS = Q + 1;
@@ -85,8 +94,9 @@ __global__ __launch_bounds__(192) void kz_zinv(const C<T>* __restrict__ A, C<T>*
                                                long long site_stride, const C<T>* __restrict__ bvec,
                                                const C<T>* __restrict__ d0, const C<T>* __restrict__ d1, cd* partial,
                                                const SlotCtl* ctl, int need, int nwg) {
-  using Z = ZTile<L>;
+  using Z = ZTile<L, true>;
   constexpr int P = Z::P, Q = Z::Q, R = Z::R, RS = Z::RS, NT = Z::NT;
+  static_assert(Q % 2 == 0, "kept outputs n2 < Q/2");
   extern __shared__ __align__(16) unsigned char smem[];
   C<T>* tw = reinterpret_cast<C<T>*>(smem);
   C<T>* dat = tw + L;
