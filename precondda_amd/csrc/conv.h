@@ -63,7 +63,6 @@ struct ConvPlanBase {
   int f_ncol = 0;   // P2/P4 columns per tile (power of 2)
   int f_wz = 0;     // P3 kz per tile (power of 2)
   int f_xv = 0;     // P3 register-product kernel: columns per tile (0: LDS Stockham P3)
-  int f_xorder = 0; // register P3 tile order: 0 slot fastest, 1 kz block fastest
   int f_zr = 0;     // register z kernels (whole rows, one LDS exchange): bit 0 P1, bit 1 P5
   size_t f_lds[5] = {0, 0, 0, 0, 0};
 };

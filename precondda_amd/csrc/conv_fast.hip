@@ -393,8 +393,7 @@ template <typename T, int L, int V, bool ST> void launch_x(ConvPlanBase* p, cons
     nblk = std::max(8LL, (nblk + 7) / 8 * 8);
     const T scale = T(1.0 / ((double)g.Lx * g.Ly * g.Lz));
     hipLaunchKernelGGL((kx_xconv<T, L, V, ST>), dim3((unsigned)nblk), dim3(NT), lds, s, (C<T>*)p->work_b,
-                       (const C<T>*)a.table, a.table_stride, g, (const C<T>*)p->tw[0], a.ctl, a.need, scale,
-                       p->f_xorder);
+                       (const C<T>*)a.table, a.table_stride, g, (const C<T>*)p->tw[0], a.ctl, a.need, scale);
   }
 }
 
@@ -559,8 +558,7 @@ bool fxv_of_plan(const ConvPlanBase* p) {
   return false;
 }
 
-// PDDA_P3_KERNEL=stockham|register forces the P3 kernel family (both are exact
-// restatements of the same convolution; the choice is a speed matter).
+// PDDA_AUTOTUNE_LOG=1 prints the timed candidates.
 bool autotune_log() {
   const char* e = getenv("PDDA_AUTOTUNE_LOG");
   return e && e[0] == '1';
@@ -582,6 +580,8 @@ int z_forced() {
   return 0;
 }
 
+// PDDA_P3_KERNEL=stockham|register forces the P3 kernel family (both are exact
+// restatements of the same convolution; the choice is a speed matter).
 int p3_forced() {
   const char* e = getenv("PDDA_P3_KERNEL");
   if (!e) return 0;
@@ -636,7 +636,6 @@ void fast_setup(ConvPlanBase* p) {
       p->fast[2] = 1;
     }
     p->f_xv = 0;
-    if (const char* e = getenv("PDDA_P3_ORDER")) p->f_xorder = atoi(e);
     // the register-product kernel reads and writes x rows n < Lx / 2, so it needs Lx = 2 nx
     const std::vector<int> xv = g.Lx == 2 * g.nx ? xconv_candidates(g.Lx, prec) : std::vector<int>();
     if (!xv.empty() && p3_forced() != 1) {
@@ -686,14 +685,27 @@ int fast_autotune(ConvPlanBase* p) {
   if (const char* e = getenv("PDDA_NO_AUTOTUNE"); e && e[0] == '1') return PDDA_OK;
   ConvGeom& g = p->g;
   const size_t es = p->prec ? 16 : 8;
-  hipStream_t s;
-  PDDA_HIP_CHECK(hipStreamCreate(&s));
-  hipEvent_t e0, e1;
-  PDDA_HIP_CHECK(hipEventCreate(&e0));
-  PDDA_HIP_CHECK(hipEventCreate(&e1));
+  // timing resources, released on every exit path
+  struct Res {
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    void* tab = nullptr;
+    ~Res() {
+      if (s) (void)hipStreamSynchronize(s);
+      if (tab) (void)hipFree(tab);
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+      if (s) (void)hipStreamDestroy(s);
+    }
+  } res;
+  PDDA_HIP_CHECK(hipStreamCreate(&res.s));
+  PDDA_HIP_CHECK(hipEventCreate(&res.e0));
+  PDDA_HIP_CHECK(hipEventCreate(&res.e1));
+  hipStream_t s = res.s;
+  hipEvent_t e0 = res.e0, e1 = res.e1;
   const size_t tab_elems = (size_t)(g.Lx / 2 + 1) * (g.Ly / 2 + 1) * (g.Lz / 2 + 1) * 6;
-  void* tab = nullptr;
-  PDDA_HIP_CHECK(hipMalloc(&tab, tab_elems * es));
+  PDDA_HIP_CHECK(hipMalloc(&res.tab, tab_elems * es));
+  void* tab = res.tab;
   // non-zero timing data (0x3c003c00 words: finite in both precisions)
   PDDA_HIP_CHECK(hipMemsetD32Async((hipDeviceptr_t)tab, 0x3c003c00, tab_elems * es / 4, s));
   PDDA_HIP_CHECK(hipMemsetD32Async((hipDeviceptr_t)p->work_a, 0x3c003c00, (size_t)g.a_stride * g.nslots * es / 4, s));
@@ -797,10 +809,6 @@ int fast_autotune(ConvPlanBase* p) {
     if (bestw > 0) p->f_lds[2] = (size_t)(g.Lx + (size_t)g.Lx * 6 * bestw) * es;
   }
   hipError_t err = hipStreamSynchronize(s);
-  (void)hipFree(tab);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  (void)hipStreamDestroy(s);
   if (err != hipSuccess) { set_error(std::string("autotune: ") + hipGetErrorString(err)); return PDDA_ERR_HIP; }
   return PDDA_OK;
 }

@@ -115,13 +115,15 @@ template <typename T> __device__ inline void load_entry(const C<T>* __restrict__
 // then the kz block interleaved with its mirror block, then the ky pair -- so that
 // tiles reading the same reduced-table rows are neighbours -- are split into 8
 // contiguous ranges, one per XCD (one L2). Each workgroup prefetches the next tile's
-// inputs into registers while it transforms the current one. The table is read from
-// L2 in step B, six components per 16-byte-aligned entry.
+// inputs into registers while it transforms the current one. The table is read in
+// step B either from L2, six components per 16-byte-aligned entry (ST = false), or from
+// an LDS copy of the tile's slice that direct global-to-LDS loads fill while step A
+// runs (ST = true).
 template <typename T, int L, int V, bool ST>
 __global__ __launch_bounds__(3 * XPlan<L>::P * V) void kx_xconv(C<T>* __restrict__ B, const C<T>* __restrict__ table,
                                                             long long table_stride, ConvGeom g,
                                                             const C<T>* __restrict__ twx, const SlotCtl* ctl,
-                                                            int need, T scale, int order) {
+                                                            int need, T scale) {
   using X = XTile<L, V>;
   constexpr int P = X::P, Q = X::Q, NT = X::NT, RS = X::RS, HX = X::HX;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -154,22 +156,12 @@ __global__ __launch_bounds__(3 * XPlan<L>::P * V) void kx_xconv(C<T>* __restrict
     Tile u;
     u.ok = lt < tend;
     if (!u.ok) return u;
-    int yr, kbp;
-    if (order == 0) {  // slot fastest, then ky / mirror, then kz block, then ky pair
-      u.slot = lt % g.nslots;
-      int r = lt / g.nslots;
-      yr = r & 1;
-      r >>= 1;
-      kbp = r % nkzb;
-      u.qy = r / nkzb;
-    } else {  // kz block fastest, then ky / mirror, then slot, then ky pair
-      kbp = lt % nkzb;
-      int r = lt / nkzb;
-      yr = r & 1;
-      r >>= 1;
-      u.slot = r % g.nslots;
-      u.qy = r / g.nslots;
-    }
+    u.slot = lt % g.nslots;
+    int r = lt / g.nslots;
+    const int yr = r & 1;
+    r >>= 1;
+    const int kbp = r % nkzb;
+    u.qy = r / nkzb;
     u.kz0 = ((kbp & 1) ? nkzb - 1 - (kbp >> 1) : (kbp >> 1)) * V;
     u.ky = yr ? g.Ly - u.qy : u.qy;
     u.ok = !(yr && (u.qy == 0 || u.qy == hy)) && slot_wanted(ctl, u.slot, need);
