@@ -90,14 +90,15 @@ template <typename T, int KIND>
 int run_reduction(const void* a, const void* b, const void* c, long long n, double* host_out, hipStream_t s) {
   constexpr int ND = KIND == 1 ? 6 : (KIND == 2 ? 2 : 1);
   double* buf = nullptr;
-  PDDA_HIP_CHECK(hipMalloc(&buf, (RNB * ND * 2 + ND * 2) * sizeof(double)));
+  PDDA_HIP_CHECK(hipMallocAsync((void**)&buf, (RNB * ND * 2 + ND * 2) * sizeof(double), s));
   hipLaunchKernelGGL((k_red_partial<T, KIND>), dim3(RNB), dim3(RNT), 0, s, (const C<T>*)a, (const C<T>*)b,
                      (const C<T>*)c, n, buf);
   hipLaunchKernelGGL(k_red_final<ND>, dim3(1), dim3(RNT), 0, s, (const double*)buf, buf + RNB * ND * 2);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpyAsync(host_out, buf + RNB * ND * 2, ND * 2 * sizeof(double), hipMemcpyDeviceToHost, s);
+  const hipError_t ef = hipFreeAsync(buf, s);
+  if (e == hipSuccess) e = ef;
   if (e == hipSuccess) e = hipStreamSynchronize(s);
-  (void)hipFree(buf);
   if (e != hipSuccess) {
     set_error(std::string("reduction: ") + hipGetErrorString(e));
     return PDDA_ERR_HIP;

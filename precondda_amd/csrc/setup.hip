@@ -4,6 +4,8 @@
 #include "../../include/pdda.h"
 #include "pdda_common.h"
 
+#include <algorithm>
+
 using namespace pdda;
 
 namespace {
@@ -37,17 +39,24 @@ __device__ inline bool inside(const double* s, double px, double py, double pz) 
 }
 #pragma clang fp contract(on)
 
-__global__ void k_mark(const double* __restrict__ shapes, int nshapes, int nx, int ny, int nz, double d,
-                       double offset, double2* __restrict__ out) {
+// Up to MARK_CHUNK shape records passed by value (no device buffer, no host sync).
+constexpr int MARK_CHUNK = 16;
+struct ShapeChunk {
+  double rec[MARK_CHUNK * SHAPE_REC];
+};
+
+// first: start from zero; otherwise later shapes overwrite what earlier chunks wrote.
+__global__ void k_mark(ShapeChunk chunk, int nshapes, int first, int nx, int ny, int nz, double d, double offset,
+                       double2* __restrict__ out) {
   const long long n = (long long)nx * ny * nz;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     const int ix = (int)(i / ((long long)ny * nz));
     const int iy = (int)((i / nz) % ny);
     const int iz = (int)(i % nz);
     const double px = (ix + offset) * d, py = (iy + offset) * d, pz = (iz + offset) * d;
-    double2 v = make_double2(0.0, 0.0);
+    double2 v = first ? make_double2(0.0, 0.0) : out[i];
     for (int q = 0; q < nshapes; ++q) {
-      const double* s = shapes + q * SHAPE_REC;
+      const double* s = chunk.rec + q * SHAPE_REC;
       if (ix < (int)s[9] || ix >= (int)s[10] || iy < (int)s[11] || iy >= (int)s[12] || iz < (int)s[13] ||
           iz >= (int)s[14])
         continue;
@@ -57,7 +66,15 @@ __global__ void k_mark(const double* __restrict__ shapes, int nshapes, int nx, i
   }
 }
 
+__global__ void k_bbox_init(int* bbox, int nx, int ny, int nz) {
+  if (threadIdx.x == 0) {
+    bbox[0] = nx; bbox[1] = ny; bbox[2] = nz;
+    bbox[3] = -1; bbox[4] = -1; bbox[5] = -1;
+  }
+}
+
 // bbox[0..2] = min index per axis, bbox[3..5] = max index per axis of nonzero sites.
+// Reduced within each wave first: one lane per wave issues the six atomics.
 __global__ void k_bbox(const double2* __restrict__ a, int nx, int ny, int nz, int* bbox) {
   const long long n = (long long)nx * ny * nz;
   int lo[3] = {nx, ny, nz}, hi[3] = {-1, -1, -1};
@@ -71,10 +88,16 @@ __global__ void k_bbox(const double2* __restrict__ a, int nx, int ny, int nz, in
       }
     }
   }
-  for (int q = 0; q < 3; ++q) {
-    atomicMin(bbox + q, lo[q]);
-    atomicMax(bbox + 3 + q, hi[q]);
-  }
+  for (int off = 32; off > 0; off >>= 1)
+    for (int q = 0; q < 3; ++q) {
+      lo[q] = min(lo[q], __shfl_xor(lo[q], off, 64));
+      hi[q] = max(hi[q], __shfl_xor(hi[q], off, 64));
+    }
+  if ((threadIdx.x & 63) == 0)
+    for (int q = 0; q < 3; ++q) {
+      atomicMin(bbox + q, lo[q]);
+      atomicMax(bbox + 3 + q, hi[q]);
+    }
 }
 
 __global__ void k_crop(const double2* __restrict__ in, int nx, int ny, int nz, int x0, int y0, int z0, int cx, int cy,
@@ -93,8 +116,14 @@ __global__ void k_crop(const double2* __restrict__ in, int nx, int ny, int nz, i
 #pragma clang fp contract(off)
 // E[r][i,j,l,c] = e_r[c] * exp(i k_r . (i,j,l) d); dirs holds (kx,ky,kz,ex,ey,ez) per rhs,
 // k already multiplied in. Phase accumulated in FP64 from integer indices.
-__global__ void k_plane_wave(const double* __restrict__ dirs, int nrhs, int nx, int ny, int nz, double d,
-                             double2* __restrict__ E) {
+// Up to WAVE_CHUNK directions passed by value (no device buffer, no host sync).
+constexpr int WAVE_CHUNK = 64;
+struct WaveChunk {
+  double v[WAVE_CHUNK * 6];
+};
+
+// E holds the chunk's first RHS at offset 0.
+__global__ void k_plane_wave(WaveChunk dirs, int nrhs, int nx, int ny, int nz, double d, double2* __restrict__ E) {
   const long long n = (long long)nx * ny * nz;
   const long long tot = n * nrhs;
   for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < tot; t += (long long)gridDim.x * blockDim.x) {
@@ -103,7 +132,7 @@ __global__ void k_plane_wave(const double* __restrict__ dirs, int nrhs, int nx, 
     const int ix = (int)(i / ((long long)ny * nz));
     const int iy = (int)((i / nz) % ny);
     const int iz = (int)(i % nz);
-    const double* q = dirs + 6 * r;
+    const double* q = dirs.v + 6 * r;
     const double ph = q[0] * (ix * d) + q[1] * (iy * d) + q[2] * (iz * d);
     double sn, cs;
     sincos(ph, &sn, &cs);
@@ -124,19 +153,15 @@ int pdda_mark_shapes(const double* shapes_host, int nshapes, int nx, int ny, int
     return PDDA_ERR_ARG;
   }
   hipStream_t s = (hipStream_t)stream;
-  double* dsh = nullptr;
-  const size_t bytes = (size_t)(nshapes > 0 ? nshapes : 1) * SHAPE_REC * sizeof(double);
-  PDDA_HIP_CHECK(hipMalloc(&dsh, bytes));
-  hipError_t e = hipMemcpyAsync(dsh, shapes_host, (size_t)nshapes * SHAPE_REC * sizeof(double),
-                                hipMemcpyHostToDevice, s);
-  if (e == hipSuccess) {
-    hipLaunchKernelGGL(k_mark, dim3(2048), dim3(256), 0, s, (const double*)dsh, nshapes, nx, ny, nz, d,
+  // records go in kernel arguments, MARK_CHUNK at a time, in shape order
+  for (int q0 = 0; q0 == 0 || q0 < nshapes; q0 += MARK_CHUNK) {
+    ShapeChunk ch{};
+    const int nq = std::min(MARK_CHUNK, nshapes - q0);
+    for (int i = 0; i < nq * SHAPE_REC; ++i) ch.rec[i] = shapes_host[(size_t)q0 * SHAPE_REC + i];
+    hipLaunchKernelGGL(k_mark, dim3(2048), dim3(256), 0, s, ch, std::max(nq, 0), q0 == 0 ? 1 : 0, nx, ny, nz, d,
                        voxel_center ? 0.5 : 0.0, (double2*)out_c128);
-    e = hipGetLastError();
+    PDDA_LAUNCH_CHECK();
   }
-  if (e == hipSuccess) e = hipStreamSynchronize(s);
-  (void)hipFree(dsh);
-  if (e != hipSuccess) { set_error(std::string("pdda_mark_shapes: ") + hipGetErrorString(e)); return PDDA_ERR_HIP; }
   return PDDA_OK;
 }
 
@@ -144,16 +169,15 @@ int pdda_trim_bbox(const void* a_c128, int nx, int ny, int nz, int* bbox_host, v
   if (!a_c128 || !bbox_host) { set_error("pdda_trim_bbox: bad arguments"); return PDDA_ERR_ARG; }
   hipStream_t s = (hipStream_t)stream;
   int* db = nullptr;
-  PDDA_HIP_CHECK(hipMalloc(&db, 6 * sizeof(int)));
-  int init[6] = {nx, ny, nz, -1, -1, -1};
-  hipError_t e = hipMemcpyAsync(db, init, sizeof(init), hipMemcpyHostToDevice, s);
-  if (e == hipSuccess) {
-    hipLaunchKernelGGL(k_bbox, dim3(1024), dim3(256), 0, s, (const double2*)a_c128, nx, ny, nz, db);
-    e = hipGetLastError();
-  }
+  // stream-ordered allocation: no device-wide synchronisation (other streams keep running)
+  PDDA_HIP_CHECK(hipMallocAsync((void**)&db, 6 * sizeof(int), s));
+  hipLaunchKernelGGL(k_bbox_init, dim3(1), dim3(64), 0, s, db, nx, ny, nz);
+  hipLaunchKernelGGL(k_bbox, dim3(1024), dim3(256), 0, s, (const double2*)a_c128, nx, ny, nz, db);
+  hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpyAsync(bbox_host, db, 6 * sizeof(int), hipMemcpyDeviceToHost, s);
+  const hipError_t ef = hipFreeAsync(db, s);
+  if (e == hipSuccess) e = ef;
   if (e == hipSuccess) e = hipStreamSynchronize(s);
-  (void)hipFree(db);
   if (e != hipSuccess) { set_error(std::string("pdda_trim_bbox: ") + hipGetErrorString(e)); return PDDA_ERR_HIP; }
   return PDDA_OK;
 }
@@ -175,16 +199,16 @@ int pdda_plane_wave(const double* dirs_host, int nrhs, int nx, int ny, int nz, d
                     void* stream) {
   if (!dirs_host || nrhs <= 0 || !out_c128) { set_error("pdda_plane_wave: bad arguments"); return PDDA_ERR_ARG; }
   hipStream_t s = (hipStream_t)stream;
-  double* dd = nullptr;
-  PDDA_HIP_CHECK(hipMalloc(&dd, (size_t)nrhs * 6 * sizeof(double)));
-  hipError_t e = hipMemcpyAsync(dd, dirs_host, (size_t)nrhs * 6 * sizeof(double), hipMemcpyHostToDevice, s);
-  if (e == hipSuccess) {
-    hipLaunchKernelGGL(k_plane_wave, dim3(2048), dim3(256), 0, s, (const double*)dd, nrhs, nx, ny, nz, d,
-                       (double2*)out_c128);
+  const long long n3 = 3LL * nx * ny * nz;
+  hipError_t e = hipSuccess;
+  for (int r0 = 0; r0 < nrhs && e == hipSuccess; r0 += WAVE_CHUNK) {
+    WaveChunk ch{};
+    const int nr = std::min(WAVE_CHUNK, nrhs - r0);
+    for (int i = 0; i < nr * 6; ++i) ch.v[i] = dirs_host[(size_t)r0 * 6 + i];
+    hipLaunchKernelGGL(k_plane_wave, dim3(2048), dim3(256), 0, s, ch, nr, nx, ny, nz, d,
+                       (double2*)out_c128 + (size_t)r0 * n3);
     e = hipGetLastError();
   }
-  if (e == hipSuccess) e = hipStreamSynchronize(s);
-  (void)hipFree(dd);
   if (e != hipSuccess) { set_error(std::string("pdda_plane_wave: ") + hipGetErrorString(e)); return PDDA_ERR_HIP; }
   return PDDA_OK;
 }
