@@ -81,7 +81,7 @@ def main():
 
     import precondda_amd as P
     from precondda_amd import _lib, workloads
-    from precondda_amd.operators import C64, C128, as_mask, conv_plan, site_inv_alpha
+    from precondda_amd.operators import C64, C128, ConvPlan, as_mask, conv_plan, site_inv_alpha
     from precondda_amd.solver import DeviceSolver, MixedSolver
 
     torch.cuda.init()
@@ -117,7 +117,8 @@ def main():
         torch.cuda.synchronize()
         dst.copy_(src)
         for dt, G, ia in ((C64, G32, ia32), (C128, G64, ia64)):
-            plan = conv_plan(1 if dt == C128 else 0, wl.grid, (nx, ny, nz))
+            plan = ConvPlan(1 if dt == C128 else 0, wl.grid, (nx, ny, nz))
+            plan.prepare_table(G)
             x = E64[0].to(dt).contiguous()
             out = torch.empty_like(x)
             for _ in range(20):
@@ -204,11 +205,13 @@ def main():
     peak = 8.0e12
     mv = {}
     for dt, e, name in ((C64, 8, "c64"), (C128, 16, "c128")):
-        plan = conv_plan(1 if dt == C128 else 0, wl.grid, (nx, ny, nz))
+        # the table prepared as the solvers prepare theirs (x-planar copy for the P3)
+        plan = ConvPlan(1 if dt == C128 else 0, wl.grid, (nx, ny, nz))
         x = E64[0].to(dt).contiguous()
         out = torch.empty_like(x)
         G = G64 if dt == C128 else G32
         ia = ia64 if dt == C128 else ia32
+        plan.prepare_table(G)
         plan.time_passes(_lib.PDDA_MODE_INTERACT, x, out, G, ia, mask, reps=5)
         ms5 = plan.time_passes(_lib.PDDA_MODE_INTERACT, x, out, G, ia, mask, reps=50)
         byts = matvec_bytes(N, Np, e)
@@ -220,7 +223,8 @@ def main():
                     "pass_GBps": [round(bb / (m * 1e-3) / 1e9, 1) for bb, m in zip(byts, ms5)]}
     if nrhs > 1:
         # the whole block of right-hand sides in one launch sequence (shared table)
-        plan = conv_plan(0, wl.grid, (nx, ny, nz), nrhs)
+        plan = ConvPlan(0, wl.grid, (nx, ny, nz), nrhs)
+        plan.prepare_table(G32)
         xb = E32.contiguous()
         outb = torch.empty_like(xb)
         plan.time_passes(_lib.PDDA_MODE_INTERACT, xb, outb, G32, ia32, mask, reps=2)

@@ -105,6 +105,13 @@ int pdda_conv_describe(const pdda_conv* c, int pass, char* buf, int n);
  * between the per-slot tables / site arrays (0 = shared by all slots). */
 int pdda_conv_apply(pdda_conv* c, int mode, const void* x, void* out, const void* table, long long table_stride,
                     const void* inv_alpha, const void* mask, long long site_stride, const void* b, void* stream);
+/* Snapshots an x-planar copy of `ntables` tables (table_stride elements apart; 0 = one
+ * shared table) inside the plan, when its x-convolution kernel reads that layout (a
+ * no-op otherwise). Later pdda_conv_apply / pdda_conv_time_passes calls that pass the
+ * same table pointer and stride read the copy, which is faster; after changing the
+ * table's contents, prepare again. Layout only: the results are the same either way.
+ * No counterpart in the reference (its tables are read in place). */
+int pdda_conv_prepare_table(pdda_conv* c, const void* table, long long table_stride, int ntables, void* stream);
 
 /* Runs `reps` convolutions (shared table and site arrays) and writes the mean
  * duration in milliseconds of each of the five passes to host ms5[5], timed with HIP

@@ -51,6 +51,7 @@ _SIGS = {
     "pdda_conv_workspace_bytes": (_ll, [_vp]),
     "pdda_conv_describe": (_i, [_vp, _i, _c.c_char_p, _i]),
     "pdda_conv_apply": (_i, [_vp, _i, _vp, _vp, _vp, _ll, _vp, _vp, _ll, _vp, _vp]),
+    "pdda_conv_prepare_table": (_i, [_vp, _vp, _ll, _i, _vp]),
     "pdda_conv_time_passes": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _i, _c.POINTER(_d), _vp]),
     "pdda_dot": (_i, [_i, _vp, _vp, _ll, _c.POINTER(_d), _vp]),
     "pdda_zeta_eta": (_i, [_i, _vp, _vp, _vp, _ll, _c.POINTER(_d), _vp]),

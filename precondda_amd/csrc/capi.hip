@@ -130,7 +130,12 @@ int bind_t(SolverT<T>& s, const void* g, long long gs, const void* m, long long 
   s.inv_alpha = (const C<T>*)ia;
   s.mask = (const uint8_t*)mask;
   s.site_stride = ss;
-  return PDDA_OK;
+  // snapshot the x-planar table copies; later launches may run on any stream
+  return guarded([&] {
+    int rc = s.prepare_tables(nullptr);
+    if (rc == PDDA_OK) PDDA_HIP_CHECK(hipStreamSynchronize(nullptr));
+    return rc;
+  });
 }
 
 template <typename T>
@@ -200,11 +205,17 @@ int pdda_conv_create(pdda_conv** out, int prec, int nx, int ny, int nz, int hx, 
   return guarded([&] {
     if (!out) { set_error("null output handle"); return PDDA_ERR_ARG; }
     auto* c = new pdda_conv();
-    int rc = conv_plan_init(&c->p, prec, nx, ny, nz, hx, hy, hz, nslots);
+    // planar: the register P3 is tuned for tables prepared with pdda_conv_prepare_table
+    int rc = conv_plan_init(&c->p, prec, nx, ny, nz, hx, hy, hz, nslots, 1);
     if (rc) { conv_plan_free(&c->p); delete c; return rc; }
     *out = c;
     return PDDA_OK;
   });
+}
+
+int pdda_conv_prepare_table(pdda_conv* c, const void* table, long long table_stride, int ntables, void* stream) {
+  if (!c || !table || ntables <= 0 || table_stride < 0) { set_error("conv_prepare_table: bad argument"); return PDDA_ERR_ARG; }
+  return guarded([&] { return conv_prepare_table(&c->p, table, table_stride, ntables, (hipStream_t)stream); });
 }
 
 int pdda_conv_destroy(pdda_conv* c) {
@@ -230,8 +241,10 @@ int pdda_conv_describe(const pdda_conv* c, int pass, char* buf, int n) {
     static const char* gen[5] = {"k_zfwd", "k_axis", "k_xconv", "k_axis", "k_zinv"};
     d = std::string(gen[pass]) + "<" + T + "> (runtime plan, L=" + std::to_string(L) + ")";
   } else if (pass == 2 && p.f_xv) {
+    // table access: 2 = the prepared x-planar copy, 1 = staged in LDS, 0 = reduced layout
+    const int tb = (p.f_xv > 0 && p.tab_x_src) ? 2 : (p.f_xv < 0 ? 1 : 0);
     d = std::string("kx_xconv<") + T + ", " + std::to_string(L) + ", " + std::to_string(p.f_xv < 0 ? -p.f_xv : p.f_xv) +
-        (p.f_xv < 0 ? ", true>" : ", false>");
+        ", " + std::to_string(tb) + ">";
   } else if (pass == 2) {
     d = std::string("kf_xconv<") + T + ", " + std::to_string(L) + "> (wz " + std::to_string(p.f_wz) + ")";
   } else if (pass == 1 || pass == 3) {
@@ -400,7 +413,11 @@ int pdda_mixed_bind(pdda_mixed* m, const void* g128, const void* g64, long long 
   m->m.ghat64 = (const double2*)g128;
   m->m.ghat64_stride = gs;
   m->m.inv_alpha64 = (const double2*)ia128;
-  return PDDA_OK;
+  return guarded([&] {
+    int rc = conv_prepare_table(&m->m.mv64, g128, gs, m->m.nslots, nullptr);
+    if (rc == PDDA_OK) PDDA_HIP_CHECK(hipStreamSynchronize(nullptr));
+    return rc;
+  });
 }
 
 int pdda_mixed_solve(pdda_mixed* m, const void* E, double tol, double inner_target, int max_outer, int max_inner,

@@ -65,7 +65,23 @@ struct ConvPlanBase {
   int f_xv = 0;     // P3 register-product kernel: columns per tile (0: LDS Stockham P3)
   int f_zr = 0;     // register z kernels (whole rows, one LDS exchange): bit 0 P1, bit 1 P5
   size_t f_lds[5] = {0, 0, 0, 0, 0};
+  // Planar plans time the register P3 on x-planar table copies (ConvArgs::table_x),
+  // and keep such a copy of their prepared table (conv_prepare_table).
+  int planar = 0;
+  void* tab_x = nullptr;           // plan-owned x-planar copy of tab_x_src
+  size_t tab_x_bytes = 0;
+  const void* tab_x_src = nullptr;  // the reduced table(s) it copies
+  long long tab_x_src_stride = 0;
+  int tab_x_n = 0;                  // tables copied (1 when shared)
 };
+
+// x-planar copy of a reduced table, read by the register P3: entry (mx, my, mz, c) at
+// ((my * (hz + 1) + mz) * 6 + c) * RX + mx, with RX = hx + 1 rounded up to 16, so that
+// the lanes of one product step (consecutive mx) read consecutive elements.
+inline int planar_row(int hx) { return (hx + 1 + 15) / 16 * 16; }
+inline long long planar_elems(int hx, int hy, int hz) { return (long long)(hy + 1) * (hz + 1) * 6 * planar_row(hx); }
+// Whether launches of this plan read x-planar tables (register P3 chosen on a planar plan).
+inline bool plan_reads_planar(const ConvPlanBase* p) { return p->planar && p->fast[2] && p->f_xv > 0; }
 
 // Arguments of one convolution launch sequence.
 struct ConvArgs {
@@ -74,6 +90,8 @@ struct ConvArgs {
   void* out = nullptr;          // box vector(s)
   const void* table = nullptr;  // reduced (hx+1,hy+1,hz+1,6) table(s)
   long long table_stride = 0;   // complex elements between slot tables (0: shared)
+  const void* table_x = nullptr;  // x-planar copies of the same tables (or null)
+  long long table_x_stride = 0;   // complex elements between slot copies (0: shared)
   const void* inv_alpha = nullptr;  // per site (nx*ny*nz), plan precision
   const uint8_t* mask = nullptr;    // per site
   long long site_stride = 0;        // elements between slot site arrays (0: shared)
@@ -98,7 +116,19 @@ int fast_autotune(ConvPlanBase* p);
 // Host API (conv.hip).
 bool fft_length_supported(int L);
 int make_fft_plan(int L, FftPlan* out);
-int conv_plan_init(ConvPlanBase* p, int prec, int nx, int ny, int nz, int hx, int hy, int hz, int nslots);
+int conv_plan_init(ConvPlanBase* p, int prec, int nx, int ny, int nz, int hx, int hy, int hz, int nslots,
+                   int planar = 0);
+// Makes the plan's x-planar copy of `table` (ntables tables, stride apart; stride 0 =
+// one shared table) when its P3 reads that layout; later launches that pass the same
+// table pointer and stride read the copy. The copy is a snapshot: after changing the
+// table's contents, prepare again (or refresh the changed slot).
+int conv_prepare_table(ConvPlanBase* p, const void* table, long long stride, int ntables, hipStream_t s);
+// Re-copies one slot of the prepared table (per-slot tables only; no-op otherwise).
+int conv_refresh_table_slot(ConvPlanBase* p, int slot, hipStream_t s);
+// Writes the x-planar copies of nslots tables (slots slot0.. of src, src_stride apart;
+// stride 0 = one shared table) into dst (dst_stride apart), for plan p's table shape.
+int table_to_planar(const ConvPlanBase* p, const void* src, long long src_stride, void* dst, long long dst_stride,
+                    int slot0, int nslots, hipStream_t s);
 void conv_plan_free(ConvPlanBase* p);
 int conv_apply(ConvPlanBase* p, const ConvArgs& a, hipStream_t s);
 // Runs `reps` convolutions and returns the mean duration of each pass P1..P5 in ms,

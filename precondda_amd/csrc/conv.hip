@@ -472,11 +472,103 @@ static int upload_twiddles(int L, int prec, void** dev) {
 
 static int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
 
-int conv_plan_init(ConvPlanBase* p, int prec, int nx, int ny, int nz, int hx, int hy, int hz, int nslots) {
+// ---------------------------------------------------------------------------------
+// x-planar table copies: (mx, my, mz, c) -> ((my * (hz+1) + mz) * 6 + c) * RX + mx, i.e.
+// the transpose of the table viewed as an (hx+1) x R matrix, R = (hy+1)(hz+1)6, padded
+// to RX columns. 32 x 16 tiles through LDS keep both sides contiguous.
+// ---------------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_table_planar(const C<T>* __restrict__ src, long long sstride,
+                                                      C<T>* __restrict__ dst, long long dstride, int hx, long long R,
+                                                      int RX, int slot0) {
+  __shared__ C<T> tile[16][33];
+  const int slot = slot0 + blockIdx.z;
+  const long long r0 = (long long)blockIdx.x * 32;
+  const int m0 = blockIdx.y * 16;
+  const C<T>* s = src + slot * sstride;
+  C<T>* d = dst + slot * dstride;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int i = ty; i < 16; i += 8) {
+    const int mx = m0 + i;
+    const long long r = r0 + tx;
+    tile[i][tx] = (mx <= hx && r < R) ? s[(long long)mx * R + r] : mk<T>(T(0), T(0));
+  }
+  __syncthreads();
+  const int ux = threadIdx.x & 15, uy = threadIdx.x >> 4;  // 16 x 16
+  for (int j = uy; j < 32; j += 16) {
+    const long long r = r0 + j;
+    if (r < R) d[r * RX + m0 + ux] = tile[ux][j];
+  }
+}
+
+int table_to_planar(const ConvPlanBase* p, const void* src, long long src_stride, void* dst, long long dst_stride,
+                    int slot0, int nslots, hipStream_t s) {
+  if (!src || !dst || nslots <= 0) { set_error("table_to_planar: bad arguments"); return PDDA_ERR_ARG; }
+  const int hx = p->g.Lx / 2, hy = p->g.Ly / 2, hz = p->g.Lz / 2;
+  const long long R = (long long)(hy + 1) * (hz + 1) * 6;
+  const int RX = planar_row(hx);
+  const dim3 grid((unsigned)((R + 31) / 32), (unsigned)(RX / 16), (unsigned)nslots);
+  if (p->prec)
+    hipLaunchKernelGGL(k_table_planar<double>, grid, dim3(256), 0, s, (const double2*)src, src_stride, (double2*)dst,
+                       dst_stride, hx, R, RX, slot0);
+  else
+    hipLaunchKernelGGL(k_table_planar<float>, grid, dim3(256), 0, s, (const float2*)src, src_stride, (float2*)dst,
+                       dst_stride, hx, R, RX, slot0);
+  PDDA_LAUNCH_CHECK();
+  return PDDA_OK;
+}
+
+int conv_prepare_table(ConvPlanBase* p, const void* table, long long stride, int ntables, hipStream_t s) {
+  if (!plan_reads_planar(p) || !table) {
+    p->tab_x_src = nullptr;
+    return PDDA_OK;
+  }
+  const int hx = p->g.Lx / 2, hy = p->g.Ly / 2, hz = p->g.Lz / 2;
+  const long long elems = planar_elems(hx, hy, hz);
+  const int n = stride ? ntables : 1;
+  if (n <= 0) { set_error("conv_prepare_table: no tables"); return PDDA_ERR_ARG; }
+  const size_t bytes = (size_t)n * elems * (p->prec ? 16 : 8);
+  if (bytes > p->tab_x_bytes) {
+    if (p->tab_x) PDDA_HIP_CHECK(hipFree(p->tab_x));
+    p->tab_x = nullptr;
+    p->tab_x_bytes = 0;
+    PDDA_HIP_CHECK(hipMalloc(&p->tab_x, bytes));
+    p->tab_x_bytes = bytes;
+  }
+  int rc = table_to_planar(p, table, stride, p->tab_x, stride ? elems : 0, 0, n, s);
+  if (rc) return rc;
+  p->tab_x_src = table;
+  p->tab_x_src_stride = stride;
+  p->tab_x_n = n;
+  return PDDA_OK;
+}
+
+int conv_refresh_table_slot(ConvPlanBase* p, int slot, hipStream_t s) {
+  if (!p->tab_x_src || p->tab_x_src_stride == 0) return PDDA_OK;
+  if (slot < 0 || slot >= p->tab_x_n) { set_error("conv_refresh_table_slot: slot out of range"); return PDDA_ERR_ARG; }
+  const long long elems = planar_elems(p->g.Lx / 2, p->g.Ly / 2, p->g.Lz / 2);
+  return table_to_planar(p, p->tab_x_src, p->tab_x_src_stride, p->tab_x, elems, slot, 1, s);
+}
+
+// The launch arguments with the plan's x-planar copy attached when `table` is the
+// prepared one.
+static ConvArgs with_planar(const ConvPlanBase* p, const ConvArgs& a) {
+  ConvArgs b = a;
+  if (!b.table_x && p->tab_x_src && a.table == p->tab_x_src && a.table_stride == p->tab_x_src_stride &&
+      plan_reads_planar(p)) {
+    b.table_x = p->tab_x;
+    b.table_x_stride = a.table_stride ? planar_elems(p->g.Lx / 2, p->g.Ly / 2, p->g.Lz / 2) : 0;
+  }
+  return b;
+}
+
+int conv_plan_init(ConvPlanBase* p, int prec, int nx, int ny, int nz, int hx, int hy, int hz, int nslots,
+                   int planar) {
   if (prec != 0 && prec != 1) { set_error("prec must be 0 (complex64) or 1 (complex128)"); return PDDA_ERR_ARG; }
   if (nx <= 0 || ny <= 0 || nz <= 0 || nslots <= 0) { set_error("box sizes and slot count must be positive"); return PDDA_ERR_ARG; }
   if (hx < nx || hy < ny || hz < nz) { set_error("table half sizes must be >= box sizes"); return PDDA_ERR_ARG; }
   p->prec = prec;
+  p->planar = planar ? 1 : 0;
   ConvGeom& g = p->g;
   g.nx = nx; g.ny = ny; g.nz = nz;
   g.Lx = 2 * hx; g.Ly = 2 * hy; g.Lz = 2 * hz;
@@ -541,10 +633,14 @@ void conv_plan_free(ConvPlanBase* p) {
   if (p->work_b) (void)hipFree(p->work_b);
   if (p->partial) (void)hipFree(p->partial);
   if (p->counters) (void)hipFree(p->counters);
+  if (p->tab_x) (void)hipFree(p->tab_x);
   p->tw[0] = p->tw[1] = p->tw[2] = nullptr;
   p->work_a = p->work_b = nullptr;
   p->partial = nullptr;
   p->counters = nullptr;
+  p->tab_x = nullptr;
+  p->tab_x_bytes = 0;
+  p->tab_x_src = nullptr;
 }
 
 template <typename T, int MODE, int DOTS, bool GEN>
@@ -687,7 +783,8 @@ int conv_apply(ConvPlanBase* p, const ConvArgs& a, hipStream_t s) {
   if (a.dots != DOT_NONE && (!a.ctl || !a.d0 || (a.dots == DOT_ZETA && !a.d1))) {
     set_error("conv_apply: fused dots need ctl and operand vectors"); return PDDA_ERR_ARG;
   }
-  return p->prec ? conv_apply_t<double>(p, a, s) : conv_apply_t<float>(p, a, s);
+  const ConvArgs b = with_planar(p, a);
+  return p->prec ? conv_apply_t<double>(p, b, s) : conv_apply_t<float>(p, b, s);
 }
 
 int conv_time_passes(ConvPlanBase* p, const ConvArgs& a, int reps, double* ms5, hipStream_t s) {
@@ -696,8 +793,9 @@ int conv_time_passes(ConvPlanBase* p, const ConvArgs& a, int reps, double* ms5, 
   for (int i = 0; i < 6; ++i) PDDA_HIP_CHECK(hipEventCreate(&ev[i]));
   for (int q = 0; q < 5; ++q) ms5[q] = 0.0;
   int rc = PDDA_OK;
+  const ConvArgs ap = with_planar(p, a);
   for (int r = 0; r < reps && rc == PDDA_OK; ++r) {
-    rc = p->prec ? conv_apply_t<double>(p, a, s, ev) : conv_apply_t<float>(p, a, s, ev);
+    rc = p->prec ? conv_apply_t<double>(p, ap, s, ev) : conv_apply_t<float>(p, ap, s, ev);
     if (rc) break;
     if (hipEventSynchronize(ev[5]) != hipSuccess) { set_error("conv_time_passes: event sync failed"); rc = PDDA_ERR_HIP; break; }
     for (int q = 0; q < 5; ++q) {

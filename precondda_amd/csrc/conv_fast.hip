@@ -375,16 +375,23 @@ bool xconv_aligned(const ConvArgs& a) {
   return ((uintptr_t)a.table & 15) == 0 && (a.table_stride & 1) == 0;
 }
 
-template <typename T, int L, int V, bool ST> void launch_x(ConvPlanBase* p, const ConvArgs& a, hipStream_t s) {
+bool autotune_log();
+
+// TB: table access of the register P3 (0 reduced layout from L2, 1 reduced layout
+// staged in LDS, 2 x-planar copy from L2).
+template <typename T, int L, int V, int TB> void launch_x(ConvPlanBase* p, const ConvArgs& a, hipStream_t s) {
   if constexpr (XPlan<L>::ok) {
     const ConvGeom& g = p->g;
     constexpr int NT = XTile<L, V>::NT;
-    const size_t lds = (size_t)XTile<L, V>::template lds_bytes<T>(ST);
+    const size_t lds = (size_t)XTile<L, V>::template lds_bytes<T>(TB == 1);
     // workgroups resident at once on the device (persistent grid)
     static const int per_cu = [&] {
       int nb = 0;
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kx_xconv<T, L, V, ST>, NT, lds) != hipSuccess || nb < 1)
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kx_xconv<T, L, V, TB>, NT, lds) != hipSuccess || nb < 1)
         nb = 1;
+      if (autotune_log())
+        fprintf(stderr, "[pdda autotune] kx_xconv<%s, %d, %d, %d>: %d workgroups per CU (%zu B LDS each)\n",
+                sizeof(T) == 4 ? "float" : "double", L, V, TB, nb, lds);
       return nb;
     }();
     const long long ntiles = (long long)g.nslots * 2 * (g.Ly / 2 + 1) * ((g.Lz + V - 1) / V);
@@ -392,23 +399,34 @@ template <typename T, int L, int V, bool ST> void launch_x(ConvPlanBase* p, cons
     long long nblk = std::min(ntiles, resident);
     nblk = std::max(8LL, (nblk + 7) / 8 * 8);
     const T scale = T(1.0 / ((double)g.Lx * g.Ly * g.Lz));
-    hipLaunchKernelGGL((kx_xconv<T, L, V, ST>), dim3((unsigned)nblk), dim3(NT), lds, s, (C<T>*)p->work_b,
-                       (const C<T>*)a.table, a.table_stride, g, (const C<T>*)p->tw[0], a.ctl, a.need, scale);
+    const void* tab = TB == 2 ? a.table_x : a.table;
+    const long long tstride = TB == 2 ? a.table_x_stride : a.table_stride;
+    hipLaunchKernelGGL((kx_xconv<T, L, V, TB>), dim3((unsigned)nblk), dim3(NT), lds, s, (C<T>*)p->work_b,
+                       (const C<T>*)tab, tstride, g, (const C<T>*)p->tw[0], a.ctl, a.need, scale);
   }
 }
 
 template <typename T, int L> void launch_p3(ConvPlanBase* p, const ConvArgs& a, hipStream_t s) {
   const ConvGeom& g = p->g;
-  if (p->f_xv && xconv_aligned(a)) {
-    // f_xv: kz per tile; a negative value selects the LDS-staged table
+  // f_xv: kz per tile of the register P3; a negative value selects the LDS-staged table
+  if (p->f_xv > 0 && a.table_x) {
     if constexpr (sizeof(T) == 4) {
-      if (p->f_xv == 8) return launch_x<T, L, 8, false>(p, a, s);
-      if (p->f_xv == 16) return launch_x<T, L, 16, false>(p, a, s);
-      if (p->f_xv == -8) return launch_x<T, L, 8, true>(p, a, s);
+      if (p->f_xv == 8) return launch_x<T, L, 8, 2>(p, a, s);
+      if (p->f_xv == 16) return launch_x<T, L, 16, 2>(p, a, s);
     } else {
-      if (p->f_xv == 4) return launch_x<T, L, 4, false>(p, a, s);
-      if (p->f_xv == 8) return launch_x<T, L, 8, false>(p, a, s);
-      if (p->f_xv == -4) return launch_x<T, L, 4, true>(p, a, s);
+      if (p->f_xv == 4) return launch_x<T, L, 4, 2>(p, a, s);
+      if (p->f_xv == 8) return launch_x<T, L, 8, 2>(p, a, s);
+    }
+  }
+  if (p->f_xv && xconv_aligned(a)) {
+    if constexpr (sizeof(T) == 4) {
+      if (p->f_xv == 8) return launch_x<T, L, 8, 0>(p, a, s);
+      if (p->f_xv == 16) return launch_x<T, L, 16, 0>(p, a, s);
+      if (p->f_xv == -8) return launch_x<T, L, 8, 1>(p, a, s);
+    } else {
+      if (p->f_xv == 4) return launch_x<T, L, 4, 0>(p, a, s);
+      if (p->f_xv == 8) return launch_x<T, L, 8, 0>(p, a, s);
+      if (p->f_xv == -4) return launch_x<T, L, 4, 1>(p, a, s);
     }
   }
   const int nblk = g.nslots * (g.Ly / 2 + 1) * ((g.Lz + p->f_wz - 1) / p->f_wz);
@@ -703,7 +721,9 @@ int fast_autotune(ConvPlanBase* p) {
   PDDA_HIP_CHECK(hipEventCreate(&res.e1));
   hipStream_t s = res.s;
   hipEvent_t e0 = res.e0, e1 = res.e1;
-  const size_t tab_elems = (size_t)(g.Lx / 2 + 1) * (g.Ly / 2 + 1) * (g.Lz / 2 + 1) * 6;
+  // one buffer serves as the reduced table and, for planar plans, as its x-planar copy
+  const size_t tab_elems = std::max((size_t)(g.Lx / 2 + 1) * (g.Ly / 2 + 1) * (g.Lz / 2 + 1) * 6,
+                                    (size_t)planar_elems(g.Lx / 2, g.Ly / 2, g.Lz / 2));
   PDDA_HIP_CHECK(hipMalloc(&res.tab, tab_elems * es));
   void* tab = res.tab;
   // non-zero timing data (0x3c003c00 words: finite in both precisions)
@@ -794,10 +814,15 @@ int fast_autotune(ConvPlanBase* p) {
     }
     const int fxv = p3_forced_xv();
     if (forced != 1 && g.Lx == 2 * g.nx) {
+      // planar plans run the register P3 on x-planar table copies (the staged variant
+      // only when forced)
+      ConvArgs ax = a;
+      if (p->planar) ax.table_x = tab;
       for (int v : xconv_candidates(g.Lx, p->prec)) {
         if (fxv && v != fxv) continue;
+        if (p->planar && v < 0 && !fxv) continue;
         p->f_xv = v;
-        const float t = time_passes(p, passes, np, a, s, e0, e1, reps);
+        const float t = time_passes(p, passes, np, ax, s, e0, e1, reps);
         if (autotune_log()) fprintf(stderr, "[pdda autotune] P3 register V=%d: %.2f us\n", v, 1e3 * t);
         if (t < best) { best = t; bestx = v; }
       }
@@ -814,7 +839,7 @@ int fast_autotune(ConvPlanBase* p) {
 }
 
 bool fast_p3_usable(const ConvPlanBase* p, const ConvArgs& a) {
-  return p->fast[2] && (p->f_wz > 0 || (p->f_xv != 0 && xconv_aligned(a)));
+  return p->fast[2] && (p->f_wz > 0 || (p->f_xv != 0 && xconv_aligned(a)) || (p->f_xv > 0 && a.table_x));
 }
 
 int fast_launch(ConvPlanBase* p, int pass, const ConvArgs& a, hipStream_t s) {

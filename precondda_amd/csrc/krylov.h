@@ -61,6 +61,10 @@ template <typename T> struct SolverT {
 
   int init(int nx, int ny, int nz, int px, int py, int pz, int nslots, int precon);
   void release();
+  // x-planar copies of the bound tables for the plans that read them (after binding),
+  // and of one slot's tables (after rebuilding them).
+  int prepare_tables(hipStream_t s);
+  int refresh_tables(int slot, hipStream_t s);
   int load(int slot, const void* E, double target, int max_iter, double ratio, int stagnation, hipStream_t s);
   int iterate(int n_iter, hipStream_t s);
   // Same iterations replayed from a captured hipGraph (n_iter even; captured on first

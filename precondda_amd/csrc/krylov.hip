@@ -313,8 +313,9 @@ int SolverT<T>::init(int nx_, int ny_, int nz_, int px_, int py_, int pz_, int n
   n3 = 3LL * nx * ny * nz;
   const int prec = sizeof(T) == 8 ? 1 : 0;
   int rc;
-  if ((rc = conv_plan_init(&mv, prec, nx, ny, nz, nx, ny, nz, nslots))) return rc;
-  if (precon && (rc = conv_plan_init(&pc, prec, nx, ny, nz, px, py, pz, nslots))) return rc;
+  // planar plans: bind() gives them x-planar copies of the bound tables
+  if ((rc = conv_plan_init(&mv, prec, nx, ny, nz, nx, ny, nz, nslots, 1))) return rc;
+  if (precon && (rc = conv_plan_init(&pc, prec, nx, ny, nz, px, py, pz, nslots, 1))) return rc;
   nwg = vec_nwg(n3);
   const size_t vb = (size_t)n3 * nslots * sizeof(C<T>);
   C<T>** bufs[] = {&vs.b, &vs.r, &vs.rhat, &vs.p, &vs.x, &vs.best, &vs.w, &vs.u, &vs.z, &vs.s0, &vs.s1,
@@ -346,6 +347,20 @@ void SolverT<T>::release() {
   if (ctl_host) (void)hipHostFree(ctl_host);
   if (partial) (void)hipFree(partial);
   if (counters) (void)hipFree(counters);
+}
+
+template <typename T>
+int SolverT<T>::prepare_tables(hipStream_t s) {
+  int rc = conv_prepare_table(&mv, ghat, ghat_stride, nslots, s);
+  if (rc == PDDA_OK && precon) rc = conv_prepare_table(&pc, mhat, mhat_stride, nslots, s);
+  return rc;
+}
+
+template <typename T>
+int SolverT<T>::refresh_tables(int slot, hipStream_t s) {
+  int rc = conv_refresh_table_slot(&mv, slot, s);
+  if (rc == PDDA_OK && precon) rc = conv_refresh_table_slot(&pc, slot, s);
+  return rc;
 }
 
 template <typename T>
@@ -525,7 +540,7 @@ int MixedSolver::init(int nx, int ny, int nz, int px, int py, int pz, int nslots
   nslots = nslots_;
   int rc;
   if ((rc = inner.init(nx, ny, nz, px, py, pz, nslots, precon))) return rc;
-  if ((rc = conv_plan_init(&mv64, 1, nx, ny, nz, nx, ny, nz, nslots))) return rc;
+  if ((rc = conv_plan_init(&mv64, 1, nx, ny, nz, nx, ny, nz, nslots, 1))) return rc;
   n3 = inner.n3;
   nwg = vec_nwg(n3);
   const size_t vb = (size_t)n3 * nslots * sizeof(double2);
@@ -654,6 +669,10 @@ int MixedSolver::slot_begin(int q, const void* E64_slot, double inner_target, in
                             int stagnation, hipStream_t s) {
   if (q < 0 || q >= nslots) { set_error("slot_begin: slot out of range"); return PDDA_ERR_ARG; }
   if (!inner.mask) { set_error("slot_begin: operands not bound"); return PDDA_ERR_STATE; }
+  // the slot's tables were just rebuilt: refresh their x-planar copies (per-slot tables)
+  int rc = inner.refresh_tables(q, s);
+  if (rc == PDDA_OK) rc = conv_refresh_table_slot(&mv64, q, s);
+  if (rc) return rc;
   // k_refine_init indexes E at slot * n3: shift the base so that the slot reads E64_slot
   const double2* Eb = (const double2*)E64_slot - (size_t)q * n3;
   hipLaunchKernelGGL(k_refine_init, dim3(nwg, 1), dim3(VNT), 0, s, Eb, inner.mask, inner.site_stride, b64, x64, r64,

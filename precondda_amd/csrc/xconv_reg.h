@@ -116,10 +116,14 @@ template <typename T> __device__ inline void load_entry(const C<T>* __restrict__
 // tiles reading the same reduced-table rows are neighbours -- are split into 8
 // contiguous ranges, one per XCD (one L2). Each workgroup prefetches the next tile's
 // inputs into registers while it transforms the current one. The table is read in
-// step B either from L2, six components per 16-byte-aligned entry (ST = false), or from
-// an LDS copy of the tile's slice that direct global-to-LDS loads fill while step A
-// runs (ST = true).
-template <typename T, int L, int V, bool ST>
+// step B in one of three ways (TB):
+//   0  from L2 in the reduced-table layout, six components per 16-byte-aligned entry;
+//   1  from an LDS copy of the tile's slice that direct global-to-LDS loads fill while
+//      step A runs;
+//   2  from L2 in the x-planar layout (planar_row): the lanes of a product step hold
+//      consecutive kx, so one load instruction touches a few cache lines instead of one
+//      line per lane.
+template <typename T, int L, int V, int TB>
 __global__ __launch_bounds__(3 * XPlan<L>::P * V) void kx_xconv(C<T>* __restrict__ B, const C<T>* __restrict__ table,
                                                             long long table_stride, ConvGeom g,
                                                             const C<T>* __restrict__ twx, const SlotCtl* ctl,
@@ -130,7 +134,7 @@ __global__ __launch_bounds__(3 * XPlan<L>::P * V) void kx_xconv(C<T>* __restrict
   C<T>* twa = reinterpret_cast<C<T>*>(smem);  // scale * w^(n k1), step A
   C<T>* twb = twa + L;                        // w^(n k1), step B
   C<T>* dat = twb + L;
-  // staged table (ST): chunk (v, mx, part) at 16-byte slot (v * (HX + 1) + mx) * E16 + part
+  // staged table (TB = 1): chunk (v, mx, part) at 16-byte slot (v * (HX + 1) + mx) * E16 + part
   constexpr int E16 = X::template chunks_per_entry<T>();
   constexpr int NCH = X::template table_chunks<T>(), NRND = X::template table_rounds<T>();
   unsigned char* tabs = reinterpret_cast<unsigned char*>(dat + V * 3 * P * RS);
@@ -194,7 +198,7 @@ __global__ __launch_bounds__(3 * XPlan<L>::P * V) void kx_xconv(C<T>* __restrict
     // orders the twiddle writes, and the previous tile's step C reads of `dat` (and step
     // B reads of the staged table), before this tile's writes
     __syncthreads();
-    if constexpr (ST) {
+    if constexpr (TB == 1) {
       if (cur.ok) {
         // asynchronous global -> LDS copy of the tile's table slice; every lane of every
         // wave takes part in each round (lanes past the slice copy chunk 0 into padding)
@@ -227,7 +231,7 @@ __global__ __launch_bounds__(3 * XPlan<L>::P * V) void kx_xconv(C<T>* __restrict
 #pragma unroll
       for (int k1 = 0; k1 < Q; ++k1) row[k1] = cmul(z[k1], twa[n1a * Q + k1]);
     }
-    if constexpr (ST) __builtin_amdgcn_s_waitcnt(0);  // this wave's table chunks have landed
+    if constexpr (TB == 1) __builtin_amdgcn_s_waitcnt(0);  // this wave's table chunks have landed
     // next tile's inputs in flight during steps B and C
     load_inputs(nxt, zc);
     __syncthreads();
@@ -246,17 +250,22 @@ __global__ __launch_bounds__(3 * XPlan<L>::P * V) void kx_xconv(C<T>* __restrict
         for (int c = 0; c < 3; ++c) DftC<T, P>::run(xv[c]);
         const bool fz = kz > hz, fy = cur.ky > hy;
         const int mz = fz ? g.Lz - kz : kz;
-        const C<T>* tg = table + cur.slot * table_stride + ((long long)cur.qy * (hz + 1) + mz) * 6;
+        constexpr int RX = (HX + 1 + 15) / 16 * 16;  // planar_row(HX)
+        const C<T>* tg = TB == 2 ? table + cur.slot * table_stride + ((long long)cur.qy * (hz + 1) + mz) * 6 * RX
+                                 : table + cur.slot * table_stride + ((long long)cur.qy * (hz + 1) + mz) * 6;
 #pragma unroll
         for (int k2 = 0; k2 < P; ++k2) {
           const int kx = k1 + Q * k2;
           const bool fx = kx > HX;
           const int mx = fx ? L - kx : kx;
           C<T> ge[6];
-          if constexpr (ST) {
+          if constexpr (TB == 1) {
             const C<T>* te = reinterpret_cast<const C<T>*>(tabs + (size_t)(v * (HX + 1) + mx) * E16 * 16);
 #pragma unroll
             for (int i = 0; i < 6; ++i) ge[i] = te[i];
+          } else if constexpr (TB == 2) {
+#pragma unroll
+            for (int i = 0; i < 6; ++i) ge[i] = tg[i * RX + mx];
           } else {
             load_entry<T>(tg + mx * mxs, ge);
           }

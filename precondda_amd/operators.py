@@ -101,6 +101,14 @@ class ConvPlan:
     def workspace_bytes(self) -> int:
         return int(lib().pdda_conv_workspace_bytes(self._h))
 
+    def prepare_table(self, table: torch.Tensor, table_stride: int = 0, ntables: int = 1, stream=None) -> None:
+        """Keeps an x-planar snapshot of `table` in the plan: later apply/time_passes calls
+        with the same tensor read it (faster x-convolution, same results). Prepare again
+        after changing the table's contents."""
+        self._prepared = table
+        check(lib().pdda_conv_prepare_table(self._h, ctypes.c_void_p(table.data_ptr()), int(table_stride),
+                                            int(ntables), stream_ptr(stream)))
+
     def apply(self, mode: int, x: torch.Tensor, out: torch.Tensor, table: torch.Tensor, inv_alpha=None,
               mask=None, b=None, table_stride: int = 0, site_stride: int = 0, stream=None) -> torch.Tensor:
         ptr = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None

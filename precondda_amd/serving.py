@@ -155,7 +155,9 @@ def slot_bytes(box) -> int:
     inner = 17 * 3 * N * c64 + 18 * N * c64           # Krylov vectors + c64 FFT workspace
     refine = 3 * 3 * N * c128 + 2 * 3 * N * c64 + 18 * N * c128  # b, x, r, e, d + c128 workspace
     operands = 6 * Np * (c64 + c128) + N * (c64 + c128 + 1) + 3 * N * c128
-    return inner + refine + operands
+    # x-planar copies of both tables (rows of nx + 1 entries padded to 16)
+    planar = (ny + 1) * (nz + 1) * 6 * ((nx + 16) // 16 * 16) * (c64 + c128)
+    return inner + refine + operands + planar
 
 
 def green_plan_bytes(box) -> int:

@@ -17,7 +17,7 @@ GRIDS = [(8, 6, 5), (12, 10, 8), (16, 16, 16), (24, 8, 12), (32, 20, 6), (48, 12
          (128, 4, 4)]
 
 
-def _run(monkeypatch, kernel, grid, prec, nslots, shared, zkernel=None):
+def _run(monkeypatch, kernel, grid, prec, nslots, shared, zkernel=None, prepare=False, desc=None):
     from precondda_amd import _lib
     from precondda_amd.operators import ConvPlan
     monkeypatch.setenv("PDDA_P3_KERNEL", kernel)
@@ -34,6 +34,10 @@ def _run(monkeypatch, kernel, grid, prec, nslots, shared, zkernel=None):
     dt = torch.complex128 if prec else torch.complex64
     tab = torch.as_tensor(np.stack([G.reshape(-1) for G in Gs])).to("cuda", dt)
     plan = ConvPlan(prec, grid, grid, nslots)
+    if prepare:
+        plan.prepare_table(tab[0] if shared else tab, 0 if shared else tab.shape[1], 1 if shared else nslots)
+    if desc is not None:
+        desc.append(plan.describe(2))
     xd = torch.as_tensor(x).to("cuda", dt)
     out = torch.empty_like(xd)
     plan.apply(_lib.PDDA_MODE_INTERACT, xd, out, tab[0] if shared else tab, inv_alpha=torch.as_tensor(ia).to("cuda", dt),
@@ -112,3 +116,57 @@ def test_z_passes_match_oracle(monkeypatch, zkernel, grid, prec):
     err = _run(monkeypatch, "stockham", grid, prec, 3, False, zkernel=zkernel)
     print(zkernel, grid, prec, "rel err", err, flush=True)
     assert err < (1e-12 if prec else 2e-6)
+
+
+@pytest.mark.parametrize("prec", [1, 0])
+@pytest.mark.parametrize("grid", GRIDS)
+def test_p3_planar_table_matches_oracle(monkeypatch, grid, prec):
+    """The register product reading the x-planar table copy (pdda_conv_prepare_table),
+    three slots with their own tables."""
+    desc = []
+    err = _run(monkeypatch, "register", grid, prec, 3, False, prepare=True, desc=desc)
+    print(desc[0], grid, prec, "rel err", err, flush=True)
+    assert desc[0].endswith(", 2>")
+    assert err < (1e-12 if prec else 2e-6)
+
+
+@pytest.mark.parametrize("prec,xv", [(0, 8), (0, 16), (1, 4), (1, 8)])
+@pytest.mark.parametrize("shared", [False, True])
+def test_p3_planar_variants(monkeypatch, prec, xv, shared):
+    monkeypatch.setenv("PDDA_P3_XV", str(xv))
+    desc = []
+    err = _run(monkeypatch, "register", (64, 6, 8), prec, 2, shared, prepare=True, desc=desc)
+    print(desc[0], "shared" if shared else "own", "rel err", err, flush=True)
+    assert desc[0] == f"kx_xconv<{'double' if prec else 'float'}, 128, {xv}, 2>"
+    assert err < (1e-12 if prec else 2e-6)
+
+
+def test_p3_planar_copy_follows_prepare(monkeypatch):
+    """The plan's x-planar copy is a snapshot: preparing again after the table changed
+    gives the product with the new table; a stockham plan ignores prepare."""
+    from precondda_amd import _lib
+    from precondda_amd.operators import ConvPlan
+    grid = (32, 12, 10)
+    n = grid[0] * grid[1] * grid[2]
+    rng = np.random.default_rng(11)
+    G1 = O.generate_interaction_row(*grid, 0.3, 1.0)
+    G2 = O.generate_interaction_row(*grid, 0.45, 1.0)
+    x = rng.standard_normal(3 * n) + 1j * rng.standard_normal(3 * n)
+    ia = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    mask = np.ones(n, dtype=np.uint8)
+    for kernel in ("register", "stockham"):
+        monkeypatch.setenv("PDDA_P3_KERNEL", kernel)
+        plan = ConvPlan(1, grid, grid, 1)
+        tab = torch.as_tensor(G1.reshape(-1)).cuda()
+        xd = torch.as_tensor(x).cuda()
+        out = torch.empty_like(xd)
+        iad = torch.as_tensor(ia).cuda()
+        md = torch.as_tensor(mask).cuda()
+        for G in (G1, G2):
+            tab.copy_(torch.as_tensor(G.reshape(-1)))
+            plan.prepare_table(tab)
+            plan.apply(_lib.PDDA_MODE_INTERACT, xd, out, tab, inv_alpha=iad, mask=md)
+            ref = O.interaction_mult(x, G, np.repeat(ia, 3), mask, grid)
+            err = np.linalg.norm(out.cpu().numpy() - ref) / np.linalg.norm(ref)
+            assert err < 1e-12, (kernel, err)
+        assert plan.describe(2).endswith(", 2>") == (kernel == "register")

@@ -15,9 +15,14 @@ def _setup(R, m, x, precon=True):
     return pb, grid, k, ia, mask, G, M
 
 
+@pytest.mark.parametrize("p3", ["auto", "register", "stockham"])
 @pytest.mark.parametrize("precon", [True, False])
-def test_fp64_solve_matches_oracle(precon):
+def test_fp64_solve_matches_oracle(monkeypatch, precon, p3):
+    """p3 = register: the x-convolution reads the x-planar copies the solver makes of its
+    tables at bind time."""
     import precondda_amd as P
+    if p3 != "auto":
+        monkeypatch.setenv("PDDA_P3_KERNEL", p3)
     pb, grid, k, ia, mask, G, M = _setup(4, 1.5 + 0.01j, 1.0)
     ref = O.solve_gpbicgstab(grid, ia, G, M, pb["E_inc"], mask, 100, 300, 1e-10, precon=precon)
     out = P.solve_gpbicgstab(grid, ia, torch.as_tensor(G).cuda(), torch.as_tensor(M).cuda(), pb["E_inc"], mask,
@@ -48,8 +53,11 @@ def test_fp64_solve_cfg1_sphere():
     assert err < 1e-10
 
 
-def test_mixed_precision_small():
+@pytest.mark.parametrize("p3", ["auto", "register"])
+def test_mixed_precision_small(monkeypatch, p3):
     import precondda_amd as P
+    if p3 != "auto":
+        monkeypatch.setenv("PDDA_P3_KERNEL", p3)
     pb, grid, k, ia, mask, G, M = _setup(6, 1.33 + 0.001j, 2.0)
     ref = O.solve_gpbicgstab(grid, ia, G, M, pb["E_inc"], mask, 100, 1000, 1e-12)
     Gd = torch.as_tensor(G).cuda()
