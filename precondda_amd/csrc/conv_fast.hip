@@ -679,19 +679,36 @@ std::vector<int> pow2_candidates(int L, int prec, int c3, int pmax) {
 
 // Milliseconds per launch sequence of the given passes: the best of three trials of
 // `reps` launches each (candidates differ by a few percent; one trial is too noisy).
+// With a flush buffer, every sequence starts after `flush_bytes` of writes have pushed
+// the plan's data out of the last-level cache, as in a solve, where the Krylov vectors
+// touched between two matvecs do the same: a kernel that wins on cache-resident data
+// can lose when its table and inputs come from HBM.
 float time_passes(ConvPlanBase* p, const int* passes, int np, const ConvArgs& a, hipStream_t s, hipEvent_t e0,
-                  hipEvent_t e1, int reps) {
+                  hipEvent_t e1, int reps, void* flush = nullptr, size_t flush_bytes = 0) {
   for (int q = 0; q < np; ++q) fast_launch(p, passes[q], a, s);  // warm-up
   float best = 1e30f;
   for (int trial = 0; trial < 3; ++trial) {
-    (void)hipEventRecord(e0, s);
-    for (int r = 0; r < reps; ++r)
-      for (int q = 0; q < np; ++q) fast_launch(p, passes[q], a, s);
-    (void)hipEventRecord(e1, s);
-    (void)hipEventSynchronize(e1);
-    float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, e0, e1);
-    best = std::min(best, ms / reps);
+    float total = 0.f;
+    if (flush) {
+      for (int r = 0; r < reps; ++r) {
+        (void)hipMemsetD32Async((hipDeviceptr_t)flush, (int)(trial * reps + r), flush_bytes / 4, s);
+        (void)hipEventRecord(e0, s);
+        for (int q = 0; q < np; ++q) fast_launch(p, passes[q], a, s);
+        (void)hipEventRecord(e1, s);
+        (void)hipEventSynchronize(e1);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        total += ms;
+      }
+    } else {
+      (void)hipEventRecord(e0, s);
+      for (int r = 0; r < reps; ++r)
+        for (int q = 0; q < np; ++q) fast_launch(p, passes[q], a, s);
+      (void)hipEventRecord(e1, s);
+      (void)hipEventSynchronize(e1);
+      (void)hipEventElapsedTime(&total, e0, e1);
+    }
+    best = std::min(best, total / reps);
   }
   return best;
 }
@@ -708,9 +725,11 @@ int fast_autotune(ConvPlanBase* p) {
     hipStream_t s = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     void* tab = nullptr;
+    void* flush = nullptr;
     ~Res() {
       if (s) (void)hipStreamSynchronize(s);
       if (tab) (void)hipFree(tab);
+      if (flush) (void)hipFree(flush);
       if (e0) (void)hipEventDestroy(e0);
       if (e1) (void)hipEventDestroy(e1);
       if (s) (void)hipStreamDestroy(s);
@@ -726,6 +745,13 @@ int fast_autotune(ConvPlanBase* p) {
                                     (size_t)planar_elems(g.Lx / 2, g.Ly / 2, g.Lz / 2));
   PDDA_HIP_CHECK(hipMalloc(&res.tab, tab_elems * es));
   void* tab = res.tab;
+  // twice the 256 MiB last-level cache; timing runs without it when it does not fit
+  const size_t flush_bytes = (size_t)512 << 20;
+  if (hipMalloc(&res.flush, flush_bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    res.flush = nullptr;
+  }
+  void* fl = res.flush;
   // non-zero timing data (0x3c003c00 words: finite in both precisions)
   PDDA_HIP_CHECK(hipMemsetD32Async((hipDeviceptr_t)tab, 0x3c003c00, tab_elems * es / 4, s));
   PDDA_HIP_CHECK(hipMemsetD32Async((hipDeviceptr_t)p->work_a, 0x3c003c00, (size_t)g.a_stride * g.nslots * es / 4, s));
@@ -759,7 +785,7 @@ int fast_autotune(ConvPlanBase* p) {
     if (zforce != 2) {
       for (int r : pow2_candidates(g.Lz, p->prec, 3, pmax)) {
         set_z_tiles(p, 0, r);
-        const float t = time_passes(p, passes, np, az, s, e0, e1, reps);
+        const float t = time_passes(p, passes, np, az, s, e0, e1, reps, fl, flush_bytes);
         if (autotune_log()) fprintf(stderr, "[pdda autotune] P1+P5 rows=%d: %.2f us\n", r, 1e3 * t);
         if (t < best) { best = t; bestr = r; bestz = 0; }
       }
@@ -768,7 +794,7 @@ int fast_autotune(ConvPlanBase* p) {
       // register P1, P5 or both, beside the best LDS row tile
       for (int zr = (zforce == 2 ? 3 : 1); zr <= 3; ++zr) {
         set_z_tiles(p, zr, bestr);
-        const float t = time_passes(p, passes, np, az, s, e0, e1, reps);
+        const float t = time_passes(p, passes, np, az, s, e0, e1, reps, fl, flush_bytes);
         if (autotune_log()) fprintf(stderr, "[pdda autotune] P1+P5 register mask %d: %.2f us\n", zr, 1e3 * t);
         if (t < best || zforce == 2) { best = t; bestz = zr; }
       }
@@ -785,7 +811,7 @@ int fast_autotune(ConvPlanBase* p) {
       if (c < 4) continue;
       p->f_ncol = c;
       p->f_lds[1] = p->f_lds[3] = (size_t)(g.Ly + (size_t)g.Ly * c) * es;
-      const float t = time_passes(p, passes, 2, a, s, e0, e1, reps);
+      const float t = time_passes(p, passes, 2, a, s, e0, e1, reps, fl, flush_bytes);
       if (t < best) { best = t; bestc = c; }
     }
     p->f_ncol = bestc;
@@ -807,7 +833,7 @@ int fast_autotune(ConvPlanBase* p) {
       for (int w : pow2_candidates(g.Lx, p->prec, 6, pmax)) {
         p->f_wz = w;
         p->f_lds[2] = (size_t)(g.Lx + (size_t)g.Lx * 6 * w) * es;
-        const float t = time_passes(p, passes, np, a, s, e0, e1, reps);
+        const float t = time_passes(p, passes, np, a, s, e0, e1, reps, fl, flush_bytes);
         if (autotune_log()) fprintf(stderr, "[pdda autotune] P3 stockham wz=%d: %.2f us\n", w, 1e3 * t);
         if (t < best) { best = t; bestw = w; bestx = 0; }
       }
@@ -822,7 +848,7 @@ int fast_autotune(ConvPlanBase* p) {
         if (fxv && v != fxv) continue;
         if (p->planar && v < 0 && !fxv) continue;
         p->f_xv = v;
-        const float t = time_passes(p, passes, np, ax, s, e0, e1, reps);
+        const float t = time_passes(p, passes, np, ax, s, e0, e1, reps, fl, flush_bytes);
         if (autotune_log()) fprintf(stderr, "[pdda autotune] P3 register V=%d: %.2f us\n", v, 1e3 * t);
         if (t < best) { best = t; bestx = v; }
       }

@@ -139,6 +139,13 @@ __global__ __launch_bounds__(3 * XPlan<L>::P * V) void kx_xconv(C<T>* __restrict
   constexpr int NCH = X::template table_chunks<T>(), NRND = X::template table_rounds<T>();
   unsigned char* tabs = reinterpret_cast<unsigned char*>(dat + V * 3 * P * RS);
 
+  // a conditional launch (residual recompute) usually finds no slot to work on: leave
+  // at once instead of walking the tile range
+  if (ctl != nullptr && need != NEED_ALL) {
+    bool any = false;
+    for (int q = 0; q < g.nslots && !any; ++q) any = slot_wanted(ctl, q, need);
+    if (!any) return;
+  }
   const int hy = g.Ly / 2, hz = g.Lz / 2;
   const int nkzb = (g.Lz + V - 1) / V;
   const int ntiles = g.nslots * 2 * nkzb * (hy + 1);
@@ -253,6 +260,16 @@ __global__ __launch_bounds__(3 * XPlan<L>::P * V) void kx_xconv(C<T>* __restrict
         constexpr int RX = (HX + 1 + 15) / 16 * 16;  // planar_row(HX)
         const C<T>* tg = TB == 2 ? table + cur.slot * table_stride + ((long long)cur.qy * (hz + 1) + mz) * 6 * RX
                                  : table + cur.slot * table_stride + ((long long)cur.qy * (hz + 1) + mz) * 6;
+        // planar table (TB = 2): left to itself the compiler issues all 8 x 6 loads up
+        // front, and the registers they hold cut occupancy. complex64 keeps one entry in
+        // flight ahead of the product that uses it; complex128 (twice the registers per
+        // entry) loads each entry just before its product.
+        constexpr bool ahead = TB == 2 && sizeof(T) == 4;
+        C<T> gn[6];
+        if constexpr (ahead) {
+#pragma unroll
+          for (int i = 0; i < 6; ++i) gn[i] = tg[i * RX + k1];
+        }
 #pragma unroll
         for (int k2 = 0; k2 < P; ++k2) {
           const int kx = k1 + Q * k2;
@@ -263,7 +280,18 @@ __global__ __launch_bounds__(3 * XPlan<L>::P * V) void kx_xconv(C<T>* __restrict
             const C<T>* te = reinterpret_cast<const C<T>*>(tabs + (size_t)(v * (HX + 1) + mx) * E16 * 16);
 #pragma unroll
             for (int i = 0; i < 6; ++i) ge[i] = te[i];
+          } else if constexpr (ahead) {
+#pragma unroll
+            for (int i = 0; i < 6; ++i) ge[i] = gn[i];
+            if (k2 + 1 < P) {
+              const int kxn = k1 + Q * (k2 + 1);
+              const int mxn = kxn > HX ? L - kxn : kxn;
+#pragma unroll
+              for (int i = 0; i < 6; ++i) gn[i] = tg[i * RX + mxn];
+            }
+            __builtin_amdgcn_sched_barrier(0);
           } else if constexpr (TB == 2) {
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int i = 0; i < 6; ++i) ge[i] = tg[i * RX + mx];
           } else {
