@@ -157,6 +157,9 @@ int pdda_plane_wave(const double* dirs_host, int nrhs, int nx, int ny, int nz, d
 int pdda_solver_create(pdda_solver** out, int prec, int nx, int ny, int nz, int px, int py, int pz, int nslots,
                        int precon);
 int pdda_solver_destroy(pdda_solver* s);
+/* Binds the operands. When the x-convolution reads x-planar tables, bind makes the
+ * solver's copies of ghat / mhat (a snapshot, see pdda_conv_prepare_table) and waits
+ * for them: bind again after changing a bound table. */
 int pdda_solver_bind(pdda_solver* s, const void* ghat, long long ghat_stride, const void* mhat,
                      long long mhat_stride, const void* inv_alpha, const void* mask, long long site_stride);
 /* Loads E_inc for one slot (slot >= 0) or all slots (slot = -1, nslots vectors) and
@@ -191,7 +194,9 @@ int pdda_mixed_result(pdda_mixed* m, void* P_out_c128, pdda_mixed_report* rep, i
  * the reference. set_serving(1) lets slots start at different iterations. */
 int pdda_mixed_set_serving(pdda_mixed* m, int on);
 /* Starts slot `slot` on E_inc (3N complex128 of that slot): b = E*mask, x = 0, r = b,
- * and loads the complex64 inner solve with r/||r||. */
+ * and loads the complex64 inner solve with r/||r||. With per-slot tables it first
+ * refreshes the slot's x-planar table copies, so the slot's tables are (re)built
+ * before this call, on the same stream. */
 int pdda_mixed_slot_begin(pdda_mixed* m, int slot, const void* E64_slot, double inner_target, int max_inner,
                           double ratio, int stagnation, void* stream);
 /* Reloads the inner solve of `slot` with the current r/||r|| (after a refinement). */
