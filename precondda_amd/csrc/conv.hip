@@ -527,6 +527,11 @@ int conv_prepare_table(ConvPlanBase* p, const void* table, long long stride, int
   const long long elems = planar_elems(hx, hy, hz);
   const int n = stride ? ntables : 1;
   if (n <= 0) { set_error("conv_prepare_table: no tables"); return PDDA_ERR_ARG; }
+  // launches read one copy per slot: per-slot tables must cover every slot of the plan
+  if (stride && n < p->g.nslots) {
+    set_error("conv_prepare_table: fewer per-slot tables than the plan's slots");
+    return PDDA_ERR_ARG;
+  }
   const size_t bytes = (size_t)n * elems * (p->prec ? 16 : 8);
   if (bytes > p->tab_x_bytes) {
     if (p->tab_x) PDDA_HIP_CHECK(hipFree(p->tab_x));

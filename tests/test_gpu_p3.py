@@ -170,3 +170,18 @@ def test_p3_planar_copy_follows_prepare(monkeypatch):
             err = np.linalg.norm(out.cpu().numpy() - ref) / np.linalg.norm(ref)
             assert err < 1e-12, (kernel, err)
         assert plan.describe(2).endswith(", 2>") == (kernel == "register")
+
+
+def test_p3_prepare_rejects_too_few_slot_tables(monkeypatch):
+    """Per-slot tables must cover every slot of the plan (the launches read one copy
+    per slot); the error comes back before anything runs."""
+    from precondda_amd._lib import PddaError
+    from precondda_amd.operators import ConvPlan
+    monkeypatch.setenv("PDDA_P3_KERNEL", "register")
+    grid = (16, 16, 16)
+    plan = ConvPlan(0, grid, grid, 3)
+    elems = 17 * 17 * 17 * 6
+    tab = torch.zeros(2 * elems, dtype=torch.complex64, device="cuda")
+    with pytest.raises(PddaError):
+        plan.prepare_table(tab, elems, 2)
+    plan.prepare_table(tab[:elems], 0, 1)  # one shared table is fine
