@@ -309,17 +309,21 @@ bool has_plan(int L) {
 
 int ept_of(int prec, int L) { return prec ? (L >= 384 ? 16 : 12) : (L >= 384 ? 32 : 24); }
 
+size_t device_lds_limit();
+
 // Largest power-of-two multiplier p <= want with C = c3 * p meeting the register bound
-// (256 / C >= ceil(L / EPT)) and the LDS budget; 0 if none.
+// (256 / C >= ceil(L / EPT)) and the LDS budget (the device's limit per workgroup:
+// 160 KiB on gfx950, so long transforms can keep wide column tiles); 0 if none.
 int pick_pow2(int L, int prec, int c3, int want, int pmax) {
   const int ept = ept_of(prec, L);
   const int need_js = (L + ept - 1) / ept;
   const size_t es = prec ? 16 : 8;
+  const size_t lds_cap = device_lds_limit();
   int best = 0;
   for (int p = 1; p <= pmax && p <= want; p *= 2) {
     const int C = c3 * p;
     if (C > FFT_NT || FFT_NT / C < need_js) break;
-    if ((size_t)(L + (size_t)L * C) * es > 60 * 1024) break;
+    if ((size_t)(L + (size_t)L * C) * es > lds_cap) break;
     best = p;
   }
   return best;
@@ -380,7 +384,7 @@ bool autotune_log();
 // TB: table access of the register P3 (0 reduced layout from L2, 1 reduced layout
 // staged in LDS, 2 x-planar copy from L2).
 template <typename T, int L, int V, int TB> void launch_x(ConvPlanBase* p, const ConvArgs& a, hipStream_t s) {
-  if constexpr (XPlan<L>::ok) {
+  if constexpr (XConvPlan<L>::ok) {
     const ConvGeom& g = p->g;
     constexpr int NT = XTile<L, V>::NT;
     const size_t lds = (size_t)XTile<L, V>::template lds_bytes<T>(TB == 1);
@@ -408,22 +412,29 @@ template <typename T, int L, int V, int TB> void launch_x(ConvPlanBase* p, const
 
 template <typename T, int L> void launch_p3(ConvPlanBase* p, const ConvArgs& a, hipStream_t s) {
   const ConvGeom& g = p->g;
-  // f_xv: kz per tile of the register P3; a negative value selects the LDS-staged table
+  // f_xv: kz per tile of the register P3; a negative value selects the LDS-staged table.
+  // Length 512 has complex64 variants V = 4, 8 only (xconv_candidates).
   if (p->f_xv > 0 && a.table_x) {
-    if constexpr (sizeof(T) == 4) {
+    if constexpr (sizeof(T) == 4 && L == 512) {
+      if (p->f_xv == 4) return launch_x<T, L, 4, 2>(p, a, s);
+      if (p->f_xv == 8) return launch_x<T, L, 8, 2>(p, a, s);
+    } else if constexpr (sizeof(T) == 4) {
       if (p->f_xv == 8) return launch_x<T, L, 8, 2>(p, a, s);
       if (p->f_xv == 16) return launch_x<T, L, 16, 2>(p, a, s);
-    } else {
+    } else if constexpr (L <= 256) {
       if (p->f_xv == 4) return launch_x<T, L, 4, 2>(p, a, s);
       if (p->f_xv == 8) return launch_x<T, L, 8, 2>(p, a, s);
     }
   }
   if (p->f_xv && xconv_aligned(a)) {
-    if constexpr (sizeof(T) == 4) {
+    if constexpr (sizeof(T) == 4 && L == 512) {
+      if (p->f_xv == 4) return launch_x<T, L, 4, 0>(p, a, s);
+      if (p->f_xv == 8) return launch_x<T, L, 8, 0>(p, a, s);
+    } else if constexpr (sizeof(T) == 4) {
       if (p->f_xv == 8) return launch_x<T, L, 8, 0>(p, a, s);
       if (p->f_xv == 16) return launch_x<T, L, 16, 0>(p, a, s);
       if (p->f_xv == -8) return launch_x<T, L, 8, 1>(p, a, s);
-    } else {
+    } else if constexpr (L <= 256) {
       if (p->f_xv == 4) return launch_x<T, L, 4, 0>(p, a, s);
       if (p->f_xv == 8) return launch_x<T, L, 8, 0>(p, a, s);
       if (p->f_xv == -4) return launch_x<T, L, 4, 1>(p, a, s);
@@ -548,10 +559,18 @@ size_t device_lds_limit() {
 std::vector<int> xconv_candidates(int L, int prec) {
   std::vector<int> out;
   int P = 0, Q = 0;
-  if (!xplan_pq(L, &P, &Q)) return out;
+  if (L == 512) {
+    // 512 = 16 * 32, complex64 only, table read from L2
+    if (prec) return out;
+    P = 16;
+    Q = 32;
+  } else if (!xplan_pq(L, &P, &Q)) {
+    return out;
+  }
   const size_t es = prec ? 16 : 8;
   // V > 0: table read from L2 in step B; V < 0: table slice staged in LDS (|V| kz)
-  const int vs[3] = {prec ? 4 : 8, prec ? 8 : 16, prec ? -4 : -8};
+  const int v512[2] = {4, 8}, vdef[3] = {prec ? 4 : 8, prec ? 8 : 16, prec ? -4 : -8};
+  const std::vector<int> vs = L == 512 ? std::vector<int>(v512, v512 + 2) : std::vector<int>(vdef, vdef + 3);
   for (int V : vs) {
     const int av = V < 0 ? -V : V;
     size_t bytes = (2 * (size_t)L + (size_t)av * 3 * P * (Q + 1)) * es;
@@ -710,6 +729,8 @@ float time_passes(ConvPlanBase* p, const int* passes, int np, const ConvArgs& a,
     }
     best = std::min(best, total / reps);
   }
+  // a candidate that failed to launch must not look fast
+  if (hipGetLastError() != hipSuccess) return 1e30f;
   return best;
 }
 
@@ -812,6 +833,7 @@ int fast_autotune(ConvPlanBase* p) {
       p->f_ncol = c;
       p->f_lds[1] = p->f_lds[3] = (size_t)(g.Ly + (size_t)g.Ly * c) * es;
       const float t = time_passes(p, passes, 2, a, s, e0, e1, reps, fl, flush_bytes);
+      if (autotune_log()) fprintf(stderr, "[pdda autotune] P2+P4 L=%d cols=%d: %.2f us\n", g.Ly, c, 1e3 * t);
       if (t < best) { best = t; bestc = c; }
     }
     p->f_ncol = bestc;

@@ -12,19 +12,28 @@
 
 namespace pdda {
 
-// cos(j pi / 24), j = 0..12: roots of unity of order 48 (covers radices 6, 12, 16).
-__host__ __device__ constexpr double kCos48[13] = {
-    1.0, 0.9914448613738104, 0.9659258262890683, 0.9238795325112867, 0.8660254037844386,
-    0.7933533402912352, 0.7071067811865476, 0.6087614290087207, 0.5, 0.3826834323650898,
-    0.25881904510252074, 0.1305261922200516, 0.0};
+// cos(j pi / 48), j = 0..24: roots of unity of order 96 (covers radices 6, 12, 16, 32).
+__host__ __device__ constexpr double kCos96[25] = {
+    1.0, 0.99785892323860350674, 0.99144486137381041115, 0.98078528040323044913, 0.96592582628906828675,
+    0.94693012949510566426, 0.92387953251128675613, 0.89687274153268830389, 0.86602540378443864676,
+    0.83146961230254523708, 0.79335334029123516458, 0.75183980747897739641, 0.70710678118654752440,
+    0.65934581510006886843, 0.60876142900872063942, 0.55557023301960222474, 0.5,
+    0.44228869021900128200, 0.38268343236508977173, 0.32143946530316158070, 0.25881904510252076235,
+    0.19509032201612826785, 0.13052619222005159155, 0.06540312923014306682, 0.0};
 
-__host__ __device__ constexpr double cos48(int a) {
-  a = ((a % 48) + 48) % 48;
-  return a <= 12 ? kCos48[a] : a <= 24 ? -kCos48[24 - a] : a <= 36 ? -kCos48[a - 24] : kCos48[48 - a];
+__host__ __device__ constexpr double cos96(int a) {
+  a = ((a % 96) + 96) % 96;
+  return a <= 24 ? kCos96[a] : a <= 48 ? -kCos96[48 - a] : a <= 72 ? -kCos96[a - 48] : kCos96[96 - a];
 }
-// exp(-2 pi i m / N) for N dividing 48
-template <int N> __host__ __device__ constexpr double root_re(int m) { return cos48(m * (48 / N)); }
-template <int N> __host__ __device__ constexpr double root_im(int m) { return -cos48(12 - m * (48 / N)); }
+// exp(-2 pi i m / N) for N dividing 96
+template <int N> __host__ __device__ constexpr double root_re(int m) {
+  static_assert(96 % N == 0, "exact roots of unity of order dividing 96 only");
+  return cos96(m * (96 / N));
+}
+template <int N> __host__ __device__ constexpr double root_im(int m) {
+  static_assert(96 % N == 0, "exact roots of unity of order dividing 96 only");
+  return -cos96(24 - m * (96 / N));
+}
 
 template <typename T, int N> struct DftC;  // forward DFT of N registers
 
@@ -64,6 +73,7 @@ template <typename T, int N1, int N2> __device__ inline void dft_compose(C<T>* v
 template <typename T> struct DftC<T, 6> { static __device__ inline void run(C<T>* v) { dft_compose<T, 3, 2>(v); } };
 template <typename T> struct DftC<T, 12> { static __device__ inline void run(C<T>* v) { dft_compose<T, 4, 3>(v); } };
 template <typename T> struct DftC<T, 16> { static __device__ inline void run(C<T>* v) { dft_compose<T, 4, 4>(v); } };
+template <typename T> struct DftC<T, 32> { static __device__ inline void run(C<T>* v) { dft_compose<T, 8, 4>(v); } };
 
 // Compile-time radix plans of the fast path.
 template <int... R> struct RList {};

@@ -74,8 +74,16 @@ PDDA_XPLAN(192, 12, 16)
 PDDA_XPLAN(256, 16, 16)
 #undef PDDA_XPLAN
 
+// Factorisations of the x-convolution: those above, and 512 = 16 * 32 (complex64 only:
+// a complex128 step B of 3 x 16 points would not leave registers for occupancy).
+template <int L> struct XConvPlan : XPlan<L> {};
+template <> struct XConvPlan<512> {
+  static constexpr bool ok = true;
+  static constexpr int P = 16, Q = 32;
+};
+
 template <int L, int V> struct XTile {
-  static constexpr int P = XPlan<L>::P, Q = XPlan<L>::Q;
+  static constexpr int P = XConvPlan<L>::P, Q = XConvPlan<L>::Q;
   static constexpr int NT = 3 * P * V;      // threads (steps A and C)
   static constexpr int RS = Q + 1;          // LDS row stride (odd: spreads banks)
   static constexpr int HX = L / 2;
@@ -124,7 +132,8 @@ template <typename T> __device__ inline void load_entry(const C<T>* __restrict__
 //      consecutive kx, so one load instruction touches a few cache lines instead of one
 //      line per lane.
 template <typename T, int L, int V, int TB>
-__global__ __launch_bounds__(3 * XPlan<L>::P * V) void kx_xconv(C<T>* __restrict__ B, const C<T>* __restrict__ table,
+__global__ __launch_bounds__(3 * XConvPlan<L>::P * V)
+__attribute__((amdgpu_waves_per_eu(L >= 512 ? 2 : 1))) void kx_xconv(C<T>* __restrict__ B, const C<T>* __restrict__ table,
                                                             long long table_stride, ConvGeom g,
                                                             const C<T>* __restrict__ twx, const SlotCtl* ctl,
                                                             int need, T scale) {
@@ -260,11 +269,12 @@ __global__ __launch_bounds__(3 * XPlan<L>::P * V) void kx_xconv(C<T>* __restrict
         constexpr int RX = (HX + 1 + 15) / 16 * 16;  // planar_row(HX)
         const C<T>* tg = TB == 2 ? table + cur.slot * table_stride + ((long long)cur.qy * (hz + 1) + mz) * 6 * RX
                                  : table + cur.slot * table_stride + ((long long)cur.qy * (hz + 1) + mz) * 6;
-        // planar table (TB = 2): left to itself the compiler issues all 8 x 6 loads up
+        // planar table (TB = 2): left to itself the compiler issues all P x 6 loads up
         // front, and the registers they hold cut occupancy. complex64 keeps one entry in
         // flight ahead of the product that uses it; complex128 (twice the registers per
-        // entry) loads each entry just before its product.
-        constexpr bool ahead = TB == 2 && sizeof(T) == 4;
+        // entry) and length 512 (16-point products) load each entry just before its
+        // product.
+        constexpr bool ahead = TB == 2 && sizeof(T) == 4 && L < 512;
         C<T> gn[6];
         if constexpr (ahead) {
 #pragma unroll

@@ -14,7 +14,7 @@ from oracle import dda_oracle as O
 pytestmark = pytest.mark.gpu
 
 GRIDS = [(8, 6, 5), (12, 10, 8), (16, 16, 16), (24, 8, 12), (32, 20, 6), (48, 12, 10), (64, 6, 8), (96, 4, 6),
-         (128, 4, 4)]
+         (128, 4, 4), (256, 4, 3)]
 
 
 def _run(monkeypatch, kernel, grid, prec, nslots, shared, zkernel=None, prepare=False, desc=None):
@@ -126,7 +126,10 @@ def test_p3_planar_table_matches_oracle(monkeypatch, grid, prec):
     desc = []
     err = _run(monkeypatch, "register", grid, prec, 3, False, prepare=True, desc=desc)
     print(desc[0], grid, prec, "rel err", err, flush=True)
-    assert desc[0].endswith(", 2>")
+    if prec and 2 * grid[0] == 512:
+        assert desc[0].startswith("kf_xconv")  # no complex128 register variant at 512
+    else:
+        assert desc[0].endswith(", 2>")
     assert err < (1e-12 if prec else 2e-6)
 
 
@@ -185,3 +188,16 @@ def test_p3_prepare_rejects_too_few_slot_tables(monkeypatch):
     with pytest.raises(PddaError):
         plan.prepare_table(tab, elems, 2)
     plan.prepare_table(tab[:elems], 0, 1)  # one shared table is fine
+
+
+@pytest.mark.parametrize("xv", [4, 8])
+@pytest.mark.parametrize("prepare", [False, True])
+def test_p3_register_512(monkeypatch, xv, prepare):
+    """Length 512 = 16 x 32 (complex64 only): both table layouts, kz blocks that do not
+    divide Lz."""
+    monkeypatch.setenv("PDDA_P3_XV", str(xv))
+    desc = []
+    err = _run(monkeypatch, "register", (256, 6, 5), 0, 2, False, prepare=prepare, desc=desc)
+    print(desc[0], "rel err", err, flush=True)
+    assert desc[0] == f"kx_xconv<float, 512, {xv}, {2 if prepare else 0}>"
+    assert err < 2e-6
