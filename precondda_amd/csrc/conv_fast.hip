@@ -446,17 +446,35 @@ template <typename T, int L> void launch_p3(ConvPlanBase* p, const ConvArgs& a, 
                      (const C<T>*)a.table, a.table_stride, g, (const C<T>*)p->tw[0], a.ctl, a.need, scale, p->f_wz);
 }
 
+template <typename T, int L, int DOTS, int MODE> void launch_kz_zinv(ConvPlanBase* p, const ConvArgs& a, hipStream_t s) {
+  const ConvGeom& g = p->g;
+  using ZI = ZTile<L, true>;
+  hipLaunchKernelGGL((kz_zinv<T, L, DOTS, MODE>), dim3((unsigned)(g.nslots * p->nwg5)), dim3(ZI::NT),
+                     (size_t)ZI::lds_elems() * sizeof(C<T>), s, (const C<T>*)p->work_a, (C<T>*)a.out, g,
+                     (const C<T>*)p->tw[2], a.mode, (const C<T>*)a.x, (const C<T>*)a.inv_alpha, a.mask,
+                     a.site_stride, (const C<T>*)a.b, (const C<T>*)a.d0, (const C<T>*)a.d1, p->partial, a.ctl,
+                     a.need, p->nwg5);
+}
+
 template <typename T, int L, int DOTS> void launch_p5d(ConvPlanBase* p, const ConvArgs& a, hipStream_t s) {
   const ConvGeom& g = p->g;
   if constexpr (XPlan<L>::ok) {
     if (p->f_zr & 2) {
-      using ZI = ZTile<L, true>;
-      hipLaunchKernelGGL((kz_zinv<T, L, DOTS>), dim3((unsigned)(g.nslots * p->nwg5)), dim3(ZI::NT),
-                         (size_t)ZI::lds_elems() * sizeof(C<T>), s, (const C<T>*)p->work_a, (C<T>*)a.out, g,
-                         (const C<T>*)p->tw[2], a.mode, (const C<T>*)a.x, (const C<T>*)a.inv_alpha, a.mask,
-                         a.site_stride, (const C<T>*)a.b, (const C<T>*)a.d0, (const C<T>*)a.d1, p->partial, a.ctl,
-                         a.need, p->nwg5);
-      return;
+      // the epilogues the solvers use are compiled per mode; others read the mode at
+      // run time
+      if constexpr (DOTS == DOT_NONE) {
+        switch (a.mode) {
+          case MODE_FIELD: return launch_kz_zinv<T, L, DOTS, MODE_FIELD>(p, a, s);
+          case MODE_PRECOND: return launch_kz_zinv<T, L, DOTS, MODE_PRECOND>(p, a, s);
+          case MODE_INTERACT: return launch_kz_zinv<T, L, DOTS, MODE_INTERACT>(p, a, s);
+          default: return launch_kz_zinv<T, L, DOTS, MODE_RESIDUAL>(p, a, s);
+        }
+      } else if constexpr (DOTS == DOT_RESID) {
+        if (a.mode == MODE_RESIDUAL) return launch_kz_zinv<T, L, DOTS, MODE_RESIDUAL>(p, a, s);
+      } else {
+        if (a.mode == MODE_INTERACT) return launch_kz_zinv<T, L, DOTS, MODE_INTERACT>(p, a, s);
+      }
+      return launch_kz_zinv<T, L, DOTS, MODE_RUNTIME>(p, a, s);
     }
   }
   const int nblk = g.nslots * p->nwg5;

@@ -87,7 +87,13 @@ __global__ __launch_bounds__(192) void kz_zfwd(const C<T>* __restrict__ x, C<T>*
   }
 }
 
-template <typename T, int L, int DOTS>
+// MODE: the epilogue (MODE_*), known at compile time so that its operands (x, alpha^-1,
+// mask, b and the dot-product vectors of this thread's kept positions) are loaded at
+// the start, in flight during the transforms; MODE_RUNTIME reads `mode` and loads them
+// after the transforms.
+constexpr int MODE_RUNTIME = -1;
+
+template <typename T, int L, int DOTS, int MODE>
 __global__ __launch_bounds__(192) void kz_zinv(const C<T>* __restrict__ A, C<T>* __restrict__ out, ConvGeom g,
                                                const C<T>* __restrict__ twz, int mode, const C<T>* __restrict__ xin,
                                                const C<T>* __restrict__ ia, const uint8_t* __restrict__ mask,
@@ -107,6 +113,37 @@ __global__ __launch_bounds__(192) void kz_zinv(const C<T>* __restrict__ A, C<T>*
   const long long row0 = (long long)wg * R;
   if (!slot_wanted(ctl, slot, need)) return;
   const int nz = g.nz, t = threadIdx.x;
+  const int md = MODE >= 0 ? MODE : mode;
+  constexpr bool PF = MODE >= 0;
+  constexpr bool use_x = MODE == MODE_INTERACT || MODE == MODE_RESIDUAL;
+  constexpr bool use_m = MODE != MODE_FIELD;
+  constexpr bool use_b = MODE == MODE_RESIDUAL;
+  constexpr bool use_d0 = DOTS == DOT_ALPHA || DOTS == DOT_RESID || DOTS == DOT_ZETA;
+  constexpr bool use_d1 = DOTS == DOT_ZETA;
+  // step C' roles (r, n1, c) and their kept positions n1 + P n2 < nz
+  const int ca = t % 3, n1a = (t / 3) % P, ra = t / (3 * P);
+  const bool livec = t < 3 * P * R && row0 + ra < rows;
+  const long long site0 = (row0 + ra) * nz;
+  const long long vbase = slot * g.vec_stride + site0 * 3 + ca;
+  const long long sbase = slot * site_stride + site0;
+  C<T> px[Q / 2], pia[Q / 2], pb[Q / 2], pd0[Q / 2], pd1[Q / 2];
+  bool pm[Q / 2];
+  if constexpr (PF) {
+#pragma unroll
+    for (int n2 = 0; n2 < Q / 2; ++n2) {
+      const int pos = n1a + P * n2;
+      if (!livec || pos >= nz) continue;
+      const long long vidx = vbase + pos * 3;
+      if constexpr (use_m) pm[n2] = mask[sbase + pos] != 0;
+      if constexpr (use_x) {
+        px[n2] = xin[vidx];
+        pia[n2] = ia[sbase + pos];
+      }
+      if constexpr (use_b) pb[n2] = bvec[vidx];
+      if constexpr (use_d0) pd0[n2] = d0[vidx];
+      if constexpr (use_d1) pd1[n2] = d1[vidx];
+    }
+  }
   for (int i = t; i < L; i += NT) tw[i] = twz[(i / Q) * (i % Q)];
   // step A': items (r, k1, c): inverse P-point DFT over k2 of A[k1 + Q k2]
   C<T> v[P];
@@ -132,16 +169,12 @@ __global__ __launch_bounds__(192) void kz_zinv(const C<T>* __restrict__ A, C<T>*
   double sre[ND], sim[ND];
 #pragma unroll
   for (int d = 0; d < ND; ++d) { sre[d] = 0.0; sim[d] = 0.0; }
-  const int ca = t % 3, n1a = (t / 3) % P, ra = t / (3 * P);
-  if (t < 3 * P * R && row0 + ra < rows) {
+  if (livec) {
     C<T> z[Q];
     const C<T>* rowp = dat + ((ra * 3 + ca) * P + n1a) * RS;
 #pragma unroll
     for (int k1 = 0; k1 < Q; ++k1) z[k1] = rowp[k1];
     DftC<T, Q>::run(z);
-    const long long site0 = (row0 + ra) * nz;
-    const long long vbase = slot * g.vec_stride + site0 * 3 + ca;
-    const long long sbase = slot * site_stride + site0;
 #pragma unroll
     for (int n2 = 0; n2 < Q / 2; ++n2) {
       const int pos = n1a + P * n2;
@@ -149,29 +182,30 @@ __global__ __launch_bounds__(192) void kz_zinv(const C<T>* __restrict__ A, C<T>*
       const C<T> cv = cconj(z[n2]);
       const long long vidx = vbase + pos * 3;
       C<T> o;
-      if (mode == MODE_FIELD) {
+      if (md == MODE_FIELD) {
         o = cv;
       } else {
-        const C<T> mc = cscale(cv, mask[sbase + pos] ? T(1) : T(0));
-        if (mode == MODE_PRECOND) {
+        const bool mk1 = PF ? pm[n2] : mask[sbase + pos] != 0;
+        const C<T> mc = cscale(cv, mk1 ? T(1) : T(0));
+        if (md == MODE_PRECOND) {
           o = mc;
         } else {
-          const C<T> w = csub(cmul(ia[sbase + pos], xin[vidx]), mc);
-          o = (mode == MODE_INTERACT) ? w : csub(bvec[vidx], w);
+          const C<T> w = csub(cmul(PF ? pia[n2] : ia[sbase + pos], PF ? px[n2] : xin[vidx]), mc);
+          o = (md == MODE_INTERACT) ? w : csub(PF ? pb[n2] : bvec[vidx], w);
         }
       }
       out[vidx] = o;
       if (DOTS == DOT_ALPHA) {
-        const C<T> a = d0[vidx];
+        const C<T> a = PF ? pd0[n2] : d0[vidx];
         sre[0] += (double)a.x * o.x + (double)a.y * o.y;
         sim[0] += (double)a.x * o.y - (double)a.y * o.x;
       } else if (DOTS == DOT_RESID) {
-        const C<T> a = d0[vidx];
+        const C<T> a = PF ? pd0[n2] : d0[vidx];
         sre[0] += (double)o.x * o.x + (double)o.y * o.y;
         sre[1] += (double)a.x * o.x + (double)a.y * o.y;
         sim[1] += (double)a.x * o.y - (double)a.y * o.x;
       } else if (DOTS == DOT_ZETA) {
-        const C<T> y = d0[vidx], s = d1[vidx];
+        const C<T> y = PF ? pd0[n2] : d0[vidx], s = PF ? pd1[n2] : d1[vidx];
         const double yx = y.x, yy = y.y, tx = o.x, ty = o.y, sx = s.x, sy = s.y;
         sre[0] += yx * yx + yy * yy;
         sre[1] += tx * tx + ty * ty;

@@ -76,9 +76,16 @@ def test_interaction_mult_matches_dense(R):
     assert err32 < 2e-6
 
 
+@pytest.mark.parametrize("zkernel", ["auto", "register", "stockham"])
 @pytest.mark.parametrize("shape", [(6, 6, 6), (10, 12, 14), (16, 16, 32)])
-def test_preconditioner_and_field_match_oracle(shape):
+def test_preconditioner_and_field_match_oracle(monkeypatch, shape, zkernel):
+    """zkernel forces the z-pass family (the register z-inverse compiles each epilogue
+    -- field, preconditioner, interaction -- separately)."""
     import precondda_amd as P
+    from precondda_amd.operators import clear_plan_cache
+    clear_plan_cache()
+    if zkernel != "auto":
+        monkeypatch.setenv("PDDA_Z_KERNEL", zkernel)
     rng = np.random.default_rng(7)
     k = 0.27
     nx, ny, nz = shape
@@ -98,6 +105,7 @@ def test_preconditioner_and_field_match_oracle(shape):
     mref = O.fft_matvec(x, G, mask, shape)
     mout = P.fft_matvec(x, None, torch.as_tensor(G).cuda(), mask, None, nx, ny, nz).cpu().numpy()
     assert np.linalg.norm(mout - mref) / np.linalg.norm(mref) < 1e-13
+    clear_plan_cache()
 
 
 def test_krylov_primitives():

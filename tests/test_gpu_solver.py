@@ -39,6 +39,24 @@ def test_fp64_solve_matches_oracle(monkeypatch, precon, p3):
     assert abs(de - ce) < 1e-10 * abs(ce) and abs(da - ca) < 1e-10 * abs(ca)
 
 
+@pytest.mark.parametrize("precon", [True, False])
+def test_fp64_solve_register_z_passes(monkeypatch, precon):
+    """The register z-inverse in every epilogue the solver launches: interaction with
+    the alpha and zeta/eta dot products, the residual with its norms (replacement at
+    iteration 1) and without (reset recompute), and the preconditioner apply."""
+    import precondda_amd as P
+    monkeypatch.setenv("PDDA_Z_KERNEL", "register")
+    pb, grid, k, ia, mask, G, M = _setup(4, 1.5 + 0.01j, 1.0)
+    ref = O.solve_gpbicgstab(grid, ia, G, M, pb["E_inc"], mask, 100, 300, 1e-10, precon=precon)
+    out = P.solve_gpbicgstab(grid, ia, torch.as_tensor(G).cuda(), torch.as_tensor(M).cuda(), pb["E_inc"], mask,
+                             100, 300, 1e-10, precon=precon)
+    print("iters", ref[0], out[0])
+    assert out[0] == ref[0]
+    err = np.linalg.norm(out[1].cpu().numpy() - ref[1]) / np.linalg.norm(ref[1])
+    print("P rel err", err)
+    assert err < 1e-10
+
+
 def test_fp64_solve_cfg1_sphere():
     """16^3 sphere, 2176 dipoles, m = 1.5+0.01i, x = 2, target 1e-8."""
     import precondda_amd as P
