@@ -20,14 +20,19 @@ const char* last_error() { return g_err.c_str(); }
 // ---------------------------------------------------------------------------------
 // Dot-product fusion helpers
 // ---------------------------------------------------------------------------------
+// DOT_ZETA sums arrive packed as four complex values: (<y,y>, <t,t>), <y,t>, <y,s>,
+// <t,s>; they are stored unpacked in sums[0..5] (<y,y>, <t,t>, <y,t>, <t,y>, <y,s>,
+// <t,s>).
 template <int ND>
 __device__ inline void finalize_dots(SlotCtl& c, int kind, const double* re, const double* im) {
-  for (int d = 0; d < ND && d < 6; ++d) c.sums[d] = cd{re[d], im[d]};
+  if (kind != DOT_ZETA)
+    for (int d = 0; d < ND && d < 6; ++d) c.sums[d] = cd{re[d], im[d]};
   if (kind == DOT_ALPHA) {
     c.alpha = cd_div(c.rho1, cd{re[0], im[0]});
-  } else if (kind == DOT_ZETA) {
-    const cd yy{re[0], im[0]}, tt{re[1], im[1]}, yt{re[2], im[2]};
-    const cd ty{re[3], im[3]}, ys{re[4], im[4]}, ts{re[5], im[5]};
+  } else if (kind == DOT_ZETA && ND >= 4) {
+    const cd yy{re[0], 0.0}, tt{im[0], 0.0}, yt{re[1], im[1]};
+    const cd ty{re[1], -im[1]}, ys{re[2], im[2]}, ts{re[3], im[3]};
+    c.sums[0] = yy; c.sums[1] = tt; c.sums[2] = yt; c.sums[3] = ty; c.sums[4] = ys; c.sums[5] = ts;
     const bool zeta_only = (c.it == 0) || (c.flags & F_WAS_RESET);
     if (zeta_only) {
       c.zeta = cd_div(ts, tt);
@@ -261,7 +266,7 @@ __global__ __launch_bounds__(FFT_NT) void k_zinv(const C<T>* __restrict__ A, C<T
   __syncthreads();
   tile_fft<T, true, GEN>(data, ncol, pz, tw);
 
-  constexpr int ND = DOTS == DOT_ALPHA ? 1 : DOTS == DOT_ZETA ? 6 : DOTS == DOT_RESID ? 2 : 1;
+  constexpr int ND = DOTS == DOT_ALPHA ? 1 : DOTS == DOT_ZETA ? 4 : DOTS == DOT_RESID ? 2 : 1;
   double sre[ND], sim[ND];
 #pragma unroll
   for (int d = 0; d < ND; ++d) { sre[d] = 0.0; sim[d] = 0.0; }
@@ -302,12 +307,12 @@ __global__ __launch_bounds__(FFT_NT) void k_zinv(const C<T>* __restrict__ A, C<T
     } else if (DOTS == DOT_ZETA) {
       const C<T> y = d0[vidx], s = d1[vidx];
       const double yx = y.x, yy = y.y, tx = o.x, ty2 = o.y, sx = s.x, sy = s.y;
-      sre[0] += yx * yx + yy * yy;                  // <y,y>
-      sre[1] += tx * tx + ty2 * ty2;                // <t,t>
-      sre[2] += yx * tx + yy * ty2; sim[2] += yx * ty2 - yy * tx;  // <y,t>
-      sre[3] += tx * yx + ty2 * yy; sim[3] += tx * yy - ty2 * yx;  // <t,y>
-      sre[4] += yx * sx + yy * sy; sim[4] += yx * sy - yy * sx;    // <y,s>
-      sre[5] += tx * sx + ty2 * sy; sim[5] += tx * sy - ty2 * sx;  // <t,s>
+      // packed: (<y,y>, <t,t>) (both real), <y,t>, <y,s>, <t,s>; <t,y> = conj(<y,t>)
+      sre[0] += yx * yx + yy * yy;
+      sim[0] += tx * tx + ty2 * ty2;
+      sre[1] += yx * tx + yy * ty2; sim[1] += yx * ty2 - yy * tx;
+      sre[2] += yx * sx + yy * sy; sim[2] += yx * sy - yy * sx;
+      sre[3] += tx * sx + ty2 * sy; sim[3] += tx * sy - ty2 * sx;
     }
   }
   if (DOTS != DOT_NONE) write_partial<ND>(sre, sim, partial + (size_t)slot * nwg * 6, wg);
@@ -682,7 +687,7 @@ static void launch_finalize(ConvPlanBase* p, const ConvArgs& a, hipStream_t s) {
                          a.ctl, (int)DOT_ALPHA, a.need, 0);
       break;
     case DOT_ZETA:
-      hipLaunchKernelGGL(k_finalize_dots<6>, dim3(nsl), dim3(FFT_NT), 0, s, (const cd*)p->partial, p->nwg5, stride,
+      hipLaunchKernelGGL(k_finalize_dots<4>, dim3(nsl), dim3(FFT_NT), 0, s, (const cd*)p->partial, p->nwg5, stride,
                          a.ctl, (int)DOT_ZETA, a.need, 0);
       break;
     case DOT_RESID:

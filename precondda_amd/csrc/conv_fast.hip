@@ -240,7 +240,7 @@ __global__ __launch_bounds__(FFT_NT) void kf_zinv(const C<T>* __restrict__ A, C<
   ct_fft<T, L, true, true>(cm, tw, lds, ldl, stN);
   __syncthreads();
 
-  constexpr int ND = DOTS == DOT_ALPHA ? 1 : DOTS == DOT_ZETA ? 6 : DOTS == DOT_RESID ? 2 : 1;
+  constexpr int ND = DOTS == DOT_ALPHA ? 1 : DOTS == DOT_ZETA ? 4 : DOTS == DOT_RESID ? 2 : 1;
   double sre[ND], sim[ND];
 #pragma unroll
   for (int d = 0; d < ND; ++d) { sre[d] = 0.0; sim[d] = 0.0; }
@@ -279,12 +279,12 @@ __global__ __launch_bounds__(FFT_NT) void kf_zinv(const C<T>* __restrict__ A, C<
     } else if (DOTS == DOT_ZETA) {
       const C<T> y = d0[vidx], s = d1[vidx];
       const double yx = y.x, yy = y.y, tx = o.x, ty2 = o.y, sx = s.x, sy = s.y;
+      // packed: (<y,y>, <t,t>) (both real), <y,t>, <y,s>, <t,s>; <t,y> = conj(<y,t>)
       sre[0] += yx * yx + yy * yy;
-      sre[1] += tx * tx + ty2 * ty2;
-      sre[2] += yx * tx + yy * ty2; sim[2] += yx * ty2 - yy * tx;
-      sre[3] += tx * yx + ty2 * yy; sim[3] += tx * yy - ty2 * yx;
-      sre[4] += yx * sx + yy * sy; sim[4] += yx * sy - yy * sx;
-      sre[5] += tx * sx + ty2 * sy; sim[5] += tx * sy - ty2 * sx;
+      sim[0] += tx * tx + ty2 * ty2;
+      sre[1] += yx * tx + yy * ty2; sim[1] += yx * ty2 - yy * tx;
+      sre[2] += yx * sx + yy * sy; sim[2] += yx * sy - yy * sx;
+      sre[3] += tx * sx + ty2 * sy; sim[3] += tx * sy - ty2 * sx;
     }
   }
   if (DOTS != DOT_NONE) {

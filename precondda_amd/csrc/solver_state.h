@@ -38,7 +38,7 @@ enum : int {
 enum : int {
   DOT_NONE = 0,
   DOT_ALPHA = 1,  // <rhat, o>            -> alpha = rho1 / sum
-  DOT_ZETA = 2,   // <y,y>,<o,o>,<y,o>,<o,y>,<y,s>,<o,s> -> zeta, eta
+  DOT_ZETA = 2,   // <y,y>,<o,o>,<y,o>,<y,s>,<o,s> (<o,y> = conj <y,o>) -> zeta, eta
   DOT_RESID = 3,  // <o,o>, <rhat,o>      -> sums[0], sums[1]
 };
 

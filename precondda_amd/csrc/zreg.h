@@ -165,7 +165,7 @@ __global__ __launch_bounds__(192) void kz_zinv(const C<T>* __restrict__ A, C<T>*
   }
   __syncthreads();
   // step C': items (r, n1, c): inverse Q-point DFT over k1, kept positions n1 + P n2 < nz
-  constexpr int ND = DOTS == DOT_ALPHA ? 1 : DOTS == DOT_ZETA ? 6 : DOTS == DOT_RESID ? 2 : 1;
+  constexpr int ND = DOTS == DOT_ALPHA ? 1 : DOTS == DOT_ZETA ? 4 : DOTS == DOT_RESID ? 2 : 1;
   double sre[ND], sim[ND];
 #pragma unroll
   for (int d = 0; d < ND; ++d) { sre[d] = 0.0; sim[d] = 0.0; }
@@ -207,12 +207,12 @@ __global__ __launch_bounds__(192) void kz_zinv(const C<T>* __restrict__ A, C<T>*
       } else if (DOTS == DOT_ZETA) {
         const C<T> y = PF ? pd0[n2] : d0[vidx], s = PF ? pd1[n2] : d1[vidx];
         const double yx = y.x, yy = y.y, tx = o.x, ty = o.y, sx = s.x, sy = s.y;
+        // packed: (<y,y>, <t,t>) (both real), <y,t>, <y,s>, <t,s>; <t,y> = conj(<y,t>)
         sre[0] += yx * yx + yy * yy;
-        sre[1] += tx * tx + ty * ty;
-        sre[2] += yx * tx + yy * ty; sim[2] += yx * ty - yy * tx;
-        sre[3] += tx * yx + ty * yy; sim[3] += tx * yy - ty * yx;
-        sre[4] += yx * sx + yy * sy; sim[4] += yx * sy - yy * sx;
-        sre[5] += tx * sx + ty * sy; sim[5] += tx * sy - ty * sx;
+        sim[0] += tx * tx + ty * ty;
+        sre[1] += yx * tx + yy * ty; sim[1] += yx * ty - yy * tx;
+        sre[2] += yx * sx + yy * sy; sim[2] += yx * sy - yy * sx;
+        sre[3] += tx * sx + ty * sy; sim[3] += tx * sy - ty * sx;
       }
     }
   }
