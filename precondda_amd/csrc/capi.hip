@@ -246,10 +246,11 @@ int pdda_conv_describe(const pdda_conv* c, int pass, char* buf, int n) {
     d = std::string("kx_xconv<") + T + ", " + std::to_string(L) + ", " + std::to_string(p.f_xv < 0 ? -p.f_xv : p.f_xv) +
         ", " + std::to_string(tb) + ">";
   } else if (pass == 2) {
-    d = std::string("kf_xconv<") + T + ", " + std::to_string(L) + "> (wz " + std::to_string(p.f_wz) + ")";
+    d = std::string("kf_xconv<") + T + ", " + std::to_string(L) + "> (wz " + std::to_string(p.f_wz) +
+        ((p.f_wide & 2) ? ", 512 threads)" : ")");
   } else if (pass == 1 || pass == 3) {
     d = std::string("kf_axis<") + T + ", " + std::to_string(L) + ", " + (pass == 3 ? "true" : "false") + "> (cols " +
-        std::to_string(p.f_ncol) + ")";
+        std::to_string(p.f_ncol) + ((p.f_wide & 1) ? ", 512 threads)" : ")");
   } else if ((pass == 0 && (p.f_zr & 1)) || (pass == 4 && (p.f_zr & 2))) {
     d = std::string(pass == 0 ? "kz_zfwd<" : "kz_zinv<") + T + ", " + std::to_string(L) + ">";
   } else {

@@ -64,6 +64,7 @@ struct ConvPlanBase {
   int f_wz = 0;     // P3 kz per tile (power of 2)
   int f_xv = 0;     // P3 register-product kernel: columns per tile (0: LDS Stockham P3)
   int f_zr = 0;     // register z kernels (whole rows, one LDS exchange): bit 0 P1, bit 1 P5
+  int f_wide = 0;   // 512-thread tiles (lengths >= 384): bit 0 P2/P4, bit 1 P3 Stockham
   size_t f_lds[5] = {0, 0, 0, 0, 0};
   // Planar plans time the register P3 on x-planar table copies (ConvArgs::table_x),
   // and keep such a copy of their prepared table (conv_prepare_table).

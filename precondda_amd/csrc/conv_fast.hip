@@ -62,8 +62,8 @@ __global__ __launch_bounds__(FFT_NT) void kf_zfwd(const C<T>* __restrict__ x, C<
 }
 
 // ------------------------------------------------------------------ P2/P4: y axis
-template <typename T, int L, bool INV>
-__global__ __launch_bounds__(FFT_NT) void kf_axis(const C<T>* __restrict__ in, C<T>* __restrict__ out, AxisArgs a,
+template <typename T, int L, bool INV, int NT = FFT_NT>
+__global__ __launch_bounds__(NT) void kf_axis(const C<T>* __restrict__ in, C<T>* __restrict__ out, AxisArgs a,
                                                   const C<T>* __restrict__ twl, const SlotCtl* ctl, int need) {
   extern __shared__ __align__(16) unsigned char smem[];
   C<T>* tw = reinterpret_cast<C<T>*>(smem);
@@ -75,7 +75,7 @@ __global__ __launch_bounds__(FFT_NT) void kf_axis(const C<T>* __restrict__ in, C
   if (!slot_wanted(ctl, o / a.outer_per_slot, need)) return;
   const int nce = (int)min((long long)a.ncol, a.W - c0);
   load_twiddles<T>(tw, twl, L);
-  const ColMap cm = ColMap::make(a.ncol);
+  const ColMap cm = ColMap::make(a.ncol, NT);
   const C<T>* src = in + (long long)o * a.in_rows * a.W + c0;
   C<T>* dst = out + (long long)o * a.out_rows * a.W + c0;
   const long long W = a.W;
@@ -88,8 +88,8 @@ __global__ __launch_bounds__(FFT_NT) void kf_axis(const C<T>* __restrict__ in, C
 }
 
 // ------------------------------------------------------------------ P3: x conv
-template <typename T, int L>
-__global__ __launch_bounds__(FFT_NT) void kf_xconv(C<T>* __restrict__ B, const C<T>* __restrict__ table,
+template <typename T, int L, int NT = FFT_NT>
+__global__ __launch_bounds__(NT) void kf_xconv(C<T>* __restrict__ B, const C<T>* __restrict__ table,
                                                    long long table_stride, ConvGeom g,
                                                    const C<T>* __restrict__ twx, const SlotCtl* ctl, int need,
                                                    T scale, int wz3) {
@@ -119,7 +119,7 @@ __global__ __launch_bounds__(FFT_NT) void kf_xconv(C<T>* __restrict__ B, const C
   C<T>* base = B + slot * g.b_stride + 3 * kz0;
   const long long y0 = (long long)qy * W, y1 = (long long)(g.Ly - qy) * W;
   load_twiddles<T>(tw, twx, L);
-  const ColMap cm = ColMap::make(nc);
+  const ColMap cm = ColMap::make(nc, NT);
   const int nx = g.nx;
   auto gaddr = [&](int pos, int col, bool& ok) -> long long {
     const int yr = col >= cw ? 1 : 0, cc = col - yr * cw;
@@ -146,7 +146,7 @@ __global__ __launch_bounds__(FFT_NT) void kf_xconv(C<T>* __restrict__ B, const C
     const int ne = (hx + 1) * wz;
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
-      const int e = threadIdx.x + u * FFT_NT;
+      const int e = threadIdx.x + u * NT;
       if (e < ne) {
         const C<T>* gp = gaddr_tab(e);
 #pragma unroll
@@ -182,10 +182,10 @@ __global__ __launch_bounds__(FFT_NT) void kf_xconv(C<T>* __restrict__ B, const C
   };
 #pragma unroll
   for (int u = 0; u < 2; ++u) {
-    const int e = threadIdx.x + u * FFT_NT;
+    const int e = threadIdx.x + u * NT;
     if (e < nent) apply(e, gpre[u]);
   }
-  for (int e = threadIdx.x + 2 * FFT_NT; e < nent; e += FFT_NT) {
+  for (int e = threadIdx.x + 2 * NT; e < nent; e += NT) {
     C<T> gq[6];
     const C<T>* gp = gaddr_tab(e);
 #pragma unroll
@@ -312,9 +312,9 @@ int ept_of(int prec, int L) { return prec ? (L >= 384 ? 16 : 12) : (L >= 384 ? 3
 size_t device_lds_limit();
 
 // Largest power-of-two multiplier p <= want with C = c3 * p meeting the register bound
-// (256 / C >= ceil(L / EPT)) and the LDS budget (the device's limit per workgroup:
+// (nt / C >= ceil(L / EPT), nt the workgroup's threads) and the LDS budget (the device's limit per workgroup:
 // 160 KiB on gfx950, so long transforms can keep wide column tiles); 0 if none.
-int pick_pow2(int L, int prec, int c3, int want, int pmax) {
+int pick_pow2(int L, int prec, int c3, int want, int pmax, int nt = FFT_NT) {
   const int ept = ept_of(prec, L);
   const int need_js = (L + ept - 1) / ept;
   const size_t es = prec ? 16 : 8;
@@ -322,7 +322,7 @@ int pick_pow2(int L, int prec, int c3, int want, int pmax) {
   int best = 0;
   for (int p = 1; p <= pmax && p <= want; p *= 2) {
     const int C = c3 * p;
-    if (C > FFT_NT || FFT_NT / C < need_js) break;
+    if (C > nt || nt / C < need_js) break;
     if ((size_t)(L + (size_t)L * C) * es > lds_cap) break;
     best = p;
   }
@@ -358,6 +358,13 @@ template <typename T, int L, bool INV> void launch_axis(ConvPlanBase* p, const C
   const long long nblk = (long long)ax.n_outer * ((ax.W + ax.ncol - 1) / ax.ncol);
   const C<T>* in = (const C<T>*)(INV ? p->work_b : p->work_a);
   C<T>* out = (C<T>*)(INV ? p->work_a : p->work_b);
+  if constexpr (L >= 384) {
+    if (p->f_wide & 1) {
+      hipLaunchKernelGGL((kf_axis<T, L, INV, 512>), dim3((unsigned)nblk), dim3(512), p->f_lds[INV ? 3 : 1], s, in,
+                         out, ax, (const C<T>*)p->tw[1], a.ctl, a.need);
+      return;
+    }
+  }
   hipLaunchKernelGGL((kf_axis<T, L, INV>), dim3((unsigned)nblk), dim3(FFT_NT), p->f_lds[INV ? 3 : 1], s, in, out,
                      ax, (const C<T>*)p->tw[1], a.ctl, a.need);
 }
@@ -442,6 +449,14 @@ template <typename T, int L> void launch_p3(ConvPlanBase* p, const ConvArgs& a, 
   }
   const int nblk = g.nslots * (g.Ly / 2 + 1) * ((g.Lz + p->f_wz - 1) / p->f_wz);
   const T scale = T(1.0 / ((double)g.Lx * g.Ly * g.Lz));
+  if constexpr (L >= 384) {
+    if (p->f_wide & 2) {
+      hipLaunchKernelGGL((kf_xconv<T, L, 512>), dim3(nblk), dim3(512), p->f_lds[2], s, (C<T>*)p->work_b,
+                         (const C<T>*)a.table, a.table_stride, g, (const C<T>*)p->tw[0], a.ctl, a.need, scale,
+                         p->f_wz);
+      return;
+    }
+  }
   hipLaunchKernelGGL((kf_xconv<T, L>), dim3(nblk), dim3(FFT_NT), p->f_lds[2], s, (C<T>*)p->work_b,
                      (const C<T>*)a.table, a.table_stride, g, (const C<T>*)p->tw[0], a.ctl, a.need, scale, p->f_wz);
 }
@@ -647,8 +662,16 @@ int p3_forced() {
 
 }  // namespace
 
+// PDDA_WIDE_TILES=1 starts lengths >= 384 on 512-thread tiles (the autotuner may keep
+// or drop them); used to test those kernels.
+bool wide_forced() {
+  const char* e = getenv("PDDA_WIDE_TILES");
+  return e && e[0] == '1';
+}
+
 void fast_setup(ConvPlanBase* p) {
   ConvGeom& g = p->g;
+  p->f_wide = 0;
   const int prec = p->prec;
   const size_t es = prec ? 16 : 8;
   const long long target = 2048;
@@ -672,8 +695,11 @@ void fast_setup(ConvPlanBase* p) {
     long long want = std::max(16LL, W * g.nx * g.nslots / target);
     int pmax = 1;
     while (pmax * 2 <= W) pmax *= 2;
-    const int c = pick_pow2(g.Ly, prec, 1, (int)std::min<long long>(want, 1 << 20), std::max(pmax, 1));
+    const bool wide = g.Ly >= 384 && wide_forced();
+    const int c = pick_pow2(g.Ly, prec, 1, (int)std::min<long long>(want, 1 << 20), std::max(pmax, 1),
+                            wide ? 512 : FFT_NT);
     if (c > 0) {
+      if (wide) p->f_wide |= 1;
       p->f_ncol = c;
       p->f_lds[1] = p->f_lds[3] = (size_t)(g.Ly + (size_t)g.Ly * c) * es;
       p->fast[1] = p->fast[3] = 1;
@@ -684,8 +710,10 @@ void fast_setup(ConvPlanBase* p) {
     long long want = std::max(1LL, (long long)g.Lz * (g.Ly / 2 + 1) * g.nslots / target);
     int pmax = 1;
     while (pmax * 2 <= g.Lz) pmax *= 2;
-    const int w = pick_pow2(g.Lx, prec, 6, (int)std::min<long long>(want, 1 << 20), pmax);
+    const bool wide = g.Lx >= 384 && wide_forced();
+    const int w = pick_pow2(g.Lx, prec, 6, (int)std::min<long long>(want, 1 << 20), pmax, wide ? 512 : FFT_NT);
     if (w > 0) {
+      if (wide) p->f_wide |= 2;
       p->f_wz = w;
       p->f_lds[2] = (size_t)(g.Lx + (size_t)g.Lx * 6 * w) * es;
       p->fast[2] = 1;
@@ -707,9 +735,9 @@ void fast_setup(ConvPlanBase* p) {
 
 namespace {
 
-std::vector<int> pow2_candidates(int L, int prec, int c3, int pmax) {
+std::vector<int> pow2_candidates(int L, int prec, int c3, int pmax, int nt = FFT_NT) {
   std::vector<int> out;
-  const int lim = pick_pow2(L, prec, c3, 1 << 20, pmax);
+  const int lim = pick_pow2(L, prec, c3, 1 << 20, pmax, nt);
   for (int q = 1; q <= lim; q *= 2) out.push_back(q);
   return out;
 }
@@ -845,16 +873,23 @@ int fast_autotune(ConvPlanBase* p) {
     while (pmax * 2 <= 3 * g.Lz) pmax *= 2;
     const int passes[2] = {1, 3};
     float best = 1e30f;
-    int bestc = p->f_ncol;
-    for (int c : pow2_candidates(g.Ly, p->prec, 1, pmax)) {
-      if (c < 4) continue;
-      p->f_ncol = c;
-      p->f_lds[1] = p->f_lds[3] = (size_t)(g.Ly + (size_t)g.Ly * c) * es;
-      const float t = time_passes(p, passes, 2, a, s, e0, e1, reps, fl, flush_bytes);
-      if (autotune_log()) fprintf(stderr, "[pdda autotune] P2+P4 L=%d cols=%d: %.2f us\n", g.Ly, c, 1e3 * t);
-      if (t < best) { best = t; bestc = c; }
+    int bestc = p->f_ncol, bestwide = p->f_wide & 1;
+    // long transforms also try 512-thread tiles, which double the columns per tile
+    for (int nt : {FFT_NT, 512}) {
+      if (nt == 512 && g.Ly < 384) break;
+      for (int c : pow2_candidates(g.Ly, p->prec, 1, pmax, nt)) {
+        if (c < 4) continue;
+        p->f_ncol = c;
+        p->f_wide = (p->f_wide & ~1) | (nt == 512 ? 1 : 0);
+        p->f_lds[1] = p->f_lds[3] = (size_t)(g.Ly + (size_t)g.Ly * c) * es;
+        const float t = time_passes(p, passes, 2, a, s, e0, e1, reps, fl, flush_bytes);
+        if (autotune_log())
+          fprintf(stderr, "[pdda autotune] P2+P4 L=%d cols=%d threads=%d: %.2f us\n", g.Ly, c, nt, 1e3 * t);
+        if (t < best) { best = t; bestc = c; bestwide = nt == 512 ? 1 : 0; }
+      }
     }
     p->f_ncol = bestc;
+    p->f_wide = (p->f_wide & ~1) | bestwide;
     p->f_lds[1] = p->f_lds[3] = (size_t)(g.Ly + (size_t)g.Ly * bestc) * es;
   }
   if (p->fast[2]) {
@@ -867,15 +902,20 @@ int fast_autotune(ConvPlanBase* p) {
       if (p->fast[q]) passes[np++] = q;
     const int forced = p3_forced();
     float best = 1e30f;
-    int bestw = p->f_wz, bestx = p->f_xv;
+    int bestw = p->f_wz, bestx = p->f_xv, bestwide = p->f_wide & 2;
     if (forced != 2 && !fxv_of_plan(p) && p->f_wz > 0) {
       p->f_xv = 0;
-      for (int w : pow2_candidates(g.Lx, p->prec, 6, pmax)) {
-        p->f_wz = w;
-        p->f_lds[2] = (size_t)(g.Lx + (size_t)g.Lx * 6 * w) * es;
-        const float t = time_passes(p, passes, np, a, s, e0, e1, reps, fl, flush_bytes);
-        if (autotune_log()) fprintf(stderr, "[pdda autotune] P3 stockham wz=%d: %.2f us\n", w, 1e3 * t);
-        if (t < best) { best = t; bestw = w; bestx = 0; }
+      for (int nt : {FFT_NT, 512}) {
+        if (nt == 512 && g.Lx < 384) break;
+        for (int w : pow2_candidates(g.Lx, p->prec, 6, pmax, nt)) {
+          p->f_wz = w;
+          p->f_wide = (p->f_wide & ~2) | (nt == 512 ? 2 : 0);
+          p->f_lds[2] = (size_t)(g.Lx + (size_t)g.Lx * 6 * w) * es;
+          const float t = time_passes(p, passes, np, a, s, e0, e1, reps, fl, flush_bytes);
+          if (autotune_log())
+            fprintf(stderr, "[pdda autotune] P3 stockham wz=%d threads=%d: %.2f us\n", w, nt, 1e3 * t);
+          if (t < best) { best = t; bestw = w; bestx = 0; bestwide = nt == 512 ? 2 : 0; }
+        }
       }
     }
     const int fxv = p3_forced_xv();
@@ -895,6 +935,7 @@ int fast_autotune(ConvPlanBase* p) {
     }
     p->f_wz = bestw;
     p->f_xv = bestx;
+    p->f_wide = (p->f_wide & ~2) | bestwide;
     if (autotune_log()) fprintf(stderr, "[pdda autotune] P3 L=%d prec=%d best %.2f us (wz %d, xv %d)\n", g.Lx, p->prec,
                                 1e3 * best, bestw, bestx);
     if (bestw > 0) p->f_lds[2] = (size_t)(g.Lx + (size_t)g.Lx * 6 * bestw) * es;

@@ -99,14 +99,15 @@ PDDA_LPLAN(512, 8, 8, 8)
 // Column map: C = c3 * 2^lg columns; thread t -> (col, jb), stride jstep = NT / C.
 struct ColMap {
   int lg, three, C, jstep;
-  __device__ static ColMap make(int C_) {
+  // nt: threads of the workgroup (FFT_NT, or a wider tile's 512)
+  __device__ static ColMap make(int C_, int nt = FFT_NT) {
     ColMap m;
     m.C = C_;
     m.three = (C_ % 3 == 0) ? 1 : 0;
     int p = m.three ? C_ / 3 : C_;
     m.lg = 0;
     while ((1 << m.lg) < p) ++m.lg;
-    m.jstep = FFT_NT / C_;
+    m.jstep = nt / C_;
     return m;
   }
   __device__ inline int div(int x) const { return three ? (x >> lg) / 3 : (x >> lg); }

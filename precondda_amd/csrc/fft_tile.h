@@ -224,7 +224,7 @@ __device__ inline void tile_fft(C<T>* data, int ncol, const FftPlan& pl, const C
 // Copies the length-L root table into LDS (caller barriers before use).
 template <typename T>
 __device__ inline void load_twiddles(C<T>* dst, const C<T>* __restrict__ src, int L) {
-  for (int i = threadIdx.x; i < L; i += FFT_NT) dst[i] = src[i];
+  for (int i = threadIdx.x; i < L; i += blockDim.x) dst[i] = src[i];
 }
 
 }  // namespace pdda

@@ -17,7 +17,7 @@ GRIDS = [(8, 6, 5), (12, 10, 8), (16, 16, 16), (24, 8, 12), (32, 20, 6), (48, 12
          (128, 4, 4), (256, 4, 3)]
 
 
-def _run(monkeypatch, kernel, grid, prec, nslots, shared, zkernel=None, prepare=False, desc=None):
+def _run(monkeypatch, kernel, grid, prec, nslots, shared, zkernel=None, prepare=False, desc=None, passes=(2,)):
     from precondda_amd import _lib
     from precondda_amd.operators import ConvPlan
     monkeypatch.setenv("PDDA_P3_KERNEL", kernel)
@@ -37,7 +37,7 @@ def _run(monkeypatch, kernel, grid, prec, nslots, shared, zkernel=None, prepare=
     if prepare:
         plan.prepare_table(tab[0] if shared else tab, 0 if shared else tab.shape[1], 1 if shared else nslots)
     if desc is not None:
-        desc.append(plan.describe(2))
+        desc.extend(plan.describe(q) for q in passes)
     xd = torch.as_tensor(x).to("cuda", dt)
     out = torch.empty_like(xd)
     plan.apply(_lib.PDDA_MODE_INTERACT, xd, out, tab[0] if shared else tab, inv_alpha=torch.as_tensor(ia).to("cuda", dt),
@@ -201,3 +201,20 @@ def test_p3_register_512(monkeypatch, xv, prepare):
     print(desc[0], "rel err", err, flush=True)
     assert desc[0] == f"kx_xconv<float, 512, {xv}, {2 if prepare else 0}>"
     assert err < 2e-6
+
+
+@pytest.mark.parametrize("prec", [1, 0])
+@pytest.mark.parametrize("grid", [(256, 4, 3), (4, 256, 3), (256, 256, 2)])
+def test_wide_tiles_512(monkeypatch, grid, prec):
+    """512-thread tiles of the y passes and of the Stockham x-convolution at length 512
+    (PDDA_WIDE_TILES with the autotuner off keeps them)."""
+    monkeypatch.setenv("PDDA_WIDE_TILES", "1")
+    monkeypatch.setenv("PDDA_NO_AUTOTUNE", "1")
+    desc = []
+    err = _run(monkeypatch, "stockham", grid, prec, 2, False, desc=desc, passes=(1, 2, 3))
+    print(desc, "rel err", err, flush=True)
+    if 2 * grid[1] == 512:
+        assert desc[0].endswith("512 threads)") and desc[2].endswith("512 threads)")
+    if 2 * grid[0] == 512:
+        assert desc[1].endswith("512 threads)")
+    assert err < (1e-12 if prec else 2e-6)
