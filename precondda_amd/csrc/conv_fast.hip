@@ -43,13 +43,13 @@ __global__ __launch_bounds__(FFT_NT) void kf_zfwd(const C<T>* __restrict__ x, C<
   const FastDiv drl = FastDiv::make((unsigned)rl);
   for (int e = threadIdx.x; e < nin; e += FFT_NT) {
     const int row = (int)drl.div((unsigned)e), r2 = e - row * rl, p = r2 / 3, c = r2 - 3 * p;
-    lds[p * C3 + row * 3 + c] = src[e];
+    lds[imul24(p, C3) + row * 3 + c] = src[e];
   }
   __syncthreads();
   const ColMap cm = ColMap::make(C3);
   const int nz = g.nz, ncv = 3 * rows;
-  auto ld0 = [&](int pos, int col) { return (pos < nz && col < ncv) ? lds[pos * C3 + col] : zero<T>(); };
-  auto stN = [&](int pos, int col, C<T> v) { lds[pos * C3 + col] = v; };
+  auto ld0 = [&](int pos, int col) { return (pos < nz && col < ncv) ? lds[imul24(pos, C3) + col] : zero<T>(); };
+  auto stN = [&](int pos, int col, C<T> v) { lds[imul24(pos, C3) + col] = v; };
   ct_fft<T, L, false, true>(cm, tw, lds, ld0, stN);
   __syncthreads();
   C<T>* dst = A + slot * g.a_stride + ((long long)ix * g.ny + y0) * L * 3;
@@ -57,7 +57,7 @@ __global__ __launch_bounds__(FFT_NT) void kf_zfwd(const C<T>* __restrict__ x, C<
   const int nout = rows * ro;
   for (int e = threadIdx.x; e < nout; e += FFT_NT) {
     const int row = e / ro, r2 = e - row * ro, p = r2 / 3, c = r2 - 3 * p;
-    dst[e] = lds[p * C3 + row * 3 + c];
+    dst[e] = lds[imul24(p, C3) + row * 3 + c];
   }
 }
 
@@ -78,16 +78,134 @@ __global__ __launch_bounds__(NT) void kf_axis(const C<T>* __restrict__ in, C<T>*
   const ColMap cm = ColMap::make(a.ncol, NT);
   const C<T>* src = in + (long long)o * a.in_rows * a.W + c0;
   C<T>* dst = out + (long long)o * a.out_rows * a.W + c0;
-  const long long W = a.W;
+  const int W = (int)a.W;  // 3 Lz <= 1536
   const int nl = a.n_load, ns = a.n_store;
-  auto ld0 = [&](int pos, int col) { return (pos < nl && col < nce) ? src[pos * W + col] : zero<T>(); };
+  auto ld0 = [&](int pos, int col) { return (pos < nl && col < nce) ? src[imul24(pos, W) + col] : zero<T>(); };
   auto stN = [&](int pos, int col, C<T> v) {
-    if (pos < ns && col < nce) dst[pos * W + col] = v;
+    if (pos < ns && col < nce) dst[imul24(pos, W) + col] = v;
   };
   ct_fft<T, L, INV, false>(cm, tw, lds, ld0, stN);
 }
 
 // ------------------------------------------------------------------ P3: x conv
+// One P3 tile: the x rows of one slot at ky = qy and its mirror Ly - qy (one row when
+// qy is 0 or Ly / 2, which are their own mirrors), kz in [kz0, kz0 + wz). Its LDS
+// columns are (row, kz, component), nc = nyr * 3 * wz3 of them.
+template <typename T> struct P3Tile {
+  int slot, qy, kz0, wz, nyr, nc;
+  int y0, y1;   // element offsets of the two ky rows within an x plane (< 3 Ly Lz)
+  C<T>* base;   // the slot's B at kz0
+};
+
+// Tile lt of the logical order (ky pair, kz block interleaved with its mirror block,
+// slot): consecutive tiles read the same reduced-table rows.
+template <typename T>
+__device__ inline P3Tile<T> xtile_at(long long lt, C<T>* B, const ConvGeom& g, int wz3, int nkzb) {
+  P3Tile<T> t;
+  t.slot = (int)(lt % g.nslots);
+  const int tq = (int)(lt / g.nslots);
+  t.qy = tq / nkzb;
+  const int kbp = tq - t.qy * nkzb;
+  const int kb = (kbp & 1) ? nkzb - 1 - (kbp >> 1) : (kbp >> 1);
+  t.nyr = (t.qy == 0 || t.qy == g.Ly / 2) ? 1 : 2;
+  t.kz0 = kb * wz3;
+  t.wz = min(wz3, g.Lz - t.kz0);
+  t.nc = t.nyr * 3 * wz3;
+  const int W = 3 * g.Lz;
+  t.y0 = t.qy * W;
+  t.y1 = (g.Ly - t.qy) * W;
+  t.base = B + t.slot * g.b_stride + 3 * t.kz0;
+  return t;
+}
+
+// Offset of LDS column col at x row pos in the tile's B (xs = 3 Ly Lz < 2^24 elements
+// per x row, and a slot's B holds fewer than 2^31); ok = false for the columns past
+// the last kz of a partial block.
+template <typename T>
+__device__ inline int xtile_addr(const P3Tile<T>& t, int pos, int col, int cw, int xs, bool& ok) {
+  const int yr = col >= cw ? 1 : 0, cc = col - yr * cw;
+  ok = cc < 3 * t.wz;
+  return imul24(pos, xs) + (yr ? t.y1 : t.y0) + cc;
+}
+
+// Reduced-table entry e = (mx, kz - kz0) of the tile: the six components of the
+// mirrored (mx, |ky|, |kz|) frequency.
+template <typename T>
+__device__ inline const C<T>* xtile_entry(const P3Tile<T>& t, const C<T>* table, long long table_stride,
+                                          const ConvGeom& g, int e, const FastDiv& dwz) {
+  const int hy = g.Ly / 2, hz = g.Lz / 2;
+  const int mx = (int)dwz.div((unsigned)e), q = e - mx * t.wz, kz = t.kz0 + q;
+  const int mz = kz > hz ? g.Lz - kz : kz;
+  const long long mxs = (long long)(hy + 1) * (hz + 1) * 6;
+  return table + t.slot * table_stride + (long long)t.qy * (hz + 1) * 6 + mx * mxs + mz * 6;
+}
+
+// 3x3 symmetric tensor product at entry e = (mx, q) of the forward-transformed tile in
+// LDS, for kx = mx and L - mx and both ky rows. An off-diagonal ab component flips sign
+// when exactly one of axes a, b is mirrored.
+template <typename T, int L>
+__device__ inline void xtile_apply(const P3Tile<T>& t, int e, const C<T> (&gq)[6], C<T>* lds, int cw, int hz, T scale,
+                                   const FastDiv& dwz) {
+  constexpr int hx = L / 2;
+  const int mx = (int)dwz.div((unsigned)e), q = e - mx * t.wz, kz = t.kz0 + q;
+  const bool fz = kz > hz;
+  const C<T> g0 = cscale(gq[0], scale), g1 = cscale(gq[1], scale), g2 = cscale(gq[2], scale);
+  const C<T> g3 = cscale(gq[3], scale), g4 = cscale(gq[4], scale), g5 = cscale(gq[5], scale);
+  const int nkx = (mx == 0 || mx == hx) ? 1 : 2;
+  for (int ix = 0; ix < nkx; ++ix) {
+    const int kx = ix ? L - mx : mx;
+    const bool fx = ix != 0;
+    for (int yr = 0; yr < t.nyr; ++yr) {
+      const bool fy = yr != 0;
+      const T sxy = (fx != fy) ? T(-1) : T(1), sxz = (fx != fz) ? T(-1) : T(1), syz = (fy != fz) ? T(-1) : T(1);
+      const C<T> h1 = cscale(g1, sxy), h2 = cscale(g2, sxz), h4 = cscale(g4, syz);
+      C<T>* dp = lds + imul24(kx, t.nc) + yr * cw + 3 * q;
+      const C<T> x0 = dp[0], x1 = dp[1], x2 = dp[2];
+      dp[0] = cadd(cadd(cmul(g0, x0), cmul(h1, x1)), cmul(h2, x2));
+      dp[1] = cadd(cadd(cmul(h1, x0), cmul(g3, x1)), cmul(h4, x2));
+      dp[2] = cadd(cadd(cmul(h2, x0), cmul(h4, x1)), cmul(g5, x2));
+    }
+  }
+}
+
+// Forward-transformed tile in LDS -> tensor product in place. The first 2 NT entries'
+// table components arrive in gpre (loaded before the forward transform so that their
+// latency overlaps it); the rest are loaded here.
+template <typename T, int L, int NT>
+__device__ inline void xtile_product(const P3Tile<T>& t, const C<T> (&gpre)[2][6], const C<T>* table,
+                                     long long table_stride, const ConvGeom& g, C<T>* lds, int cw, T scale,
+                                     const FastDiv& dwz) {
+  const int hz = g.Lz / 2;
+  const int nent = (L / 2 + 1) * t.wz;
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int e = threadIdx.x + u * NT;
+    if (e < nent) xtile_apply<T, L>(t, e, gpre[u], lds, cw, hz, scale, dwz);
+  }
+  for (int e = threadIdx.x + 2 * NT; e < nent; e += NT) {
+    C<T> gq[6];
+    const C<T>* gp = xtile_entry(t, table, table_stride, g, e, dwz);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) gq[c] = gp[c];
+    xtile_apply<T, L>(t, e, gq, lds, cw, hz, scale, dwz);
+  }
+}
+
+template <typename T, int L, int NT>
+__device__ inline void xtile_prefetch(const P3Tile<T>& t, C<T> (&gpre)[2][6], const C<T>* table, long long table_stride,
+                                      const ConvGeom& g, const FastDiv& dwz) {
+  const int ne = (L / 2 + 1) * t.wz;
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int e = threadIdx.x + u * NT;
+    if (e < ne) {
+      const C<T>* gp = xtile_entry(t, table, table_stride, g, e, dwz);
+#pragma unroll
+      for (int c = 0; c < 6; ++c) gpre[u][c] = gp[c];
+    }
+  }
+}
+
 template <typename T, int L, int NT = FFT_NT>
 __global__ __launch_bounds__(NT) void kf_xconv(C<T>* __restrict__ B, const C<T>* __restrict__ table,
                                                    long long table_stride, ConvGeom g,
@@ -96,108 +214,37 @@ __global__ __launch_bounds__(NT) void kf_xconv(C<T>* __restrict__ B, const C<T>*
   extern __shared__ __align__(16) unsigned char smem[];
   C<T>* tw = reinterpret_cast<C<T>*>(smem);
   C<T>* lds = tw + L;
-  constexpr int hx = L / 2;
-  const int hy = g.Ly / 2, hz = g.Lz / 2;
   const int nkzb = (g.Lz + wz3 - 1) / wz3;
   // XCD-aware tile order: workgroups are dispatched round-robin over the 8 XCDs, so
   // XCD x runs blocks x, x+8, ...; give each XCD a contiguous range of logical tiles
-  // ordered (ky pair, kz block interleaved with its mirror block, slot) so that tiles
-  // reading the same reduced-table rows run back to back on one L2.
+  // so that tiles reading the same reduced-table rows run back to back on one L2.
   const int nblk = gridDim.x, b = blockIdx.x, xcd = b & 7, r = b >> 3;
   const int per_xcd = nblk >> 3, remx = nblk & 7;
   const int lt = xcd * per_xcd + min(xcd, remx) + r;
-  const int slot = lt % g.nslots;
-  const int tq = lt / g.nslots;
-  const int qy = tq / nkzb, kbp = tq - qy * nkzb;
-  const int kb = (kbp & 1) ? nkzb - 1 - (kbp >> 1) : (kbp >> 1);
-  if (!slot_wanted(ctl, slot, need)) return;
-  const int nyr = (qy == 0 || qy == hy) ? 1 : 2;
-  const int kz0 = kb * wz3, wz = min(wz3, g.Lz - kz0);
-  const int cw = 3 * wz3, nc = nyr * cw, cwe = 3 * wz;
-  const long long W = 3LL * g.Lz;
-  const long long xs = (long long)g.Ly * W;
-  C<T>* base = B + slot * g.b_stride + 3 * kz0;
-  const long long y0 = (long long)qy * W, y1 = (long long)(g.Ly - qy) * W;
+  const P3Tile<T> t = xtile_at<T>(lt, B, g, wz3, nkzb);
+  if (!slot_wanted(ctl, t.slot, need)) return;
   load_twiddles<T>(tw, twx, L);
-  const ColMap cm = ColMap::make(nc, NT);
-  const int nx = g.nx;
-  auto gaddr = [&](int pos, int col, bool& ok) -> long long {
-    const int yr = col >= cw ? 1 : 0, cc = col - yr * cw;
-    ok = cc < cwe;
-    return pos * xs + (yr ? y1 : y0) + cc;
-  };
+  const ColMap cm = ColMap::make(t.nc, NT);
+  const int nx = g.nx, cw = 3 * wz3;
+  const int xs = g.Ly * 3 * g.Lz;
   auto ld0 = [&](int pos, int col) {
     bool ok;
-    const long long ad = gaddr(pos, col, ok);
-    return (pos < nx && ok) ? base[ad] : zero<T>();
+    const int ad = xtile_addr(t, pos, col, cw, xs, ok);
+    return (pos < nx && ok) ? t.base[ad] : zero<T>();
   };
-  auto stl = [&](int pos, int col, C<T> v) { lds[pos * nc + col] = v; };
-  // reduced-table entries of this tile, (hx+1) x wz, fetched before the forward FFT so
-  // that their latency overlaps it (up to two entries per thread held in registers)
-  const C<T>* tab = table + slot * table_stride + (long long)qy * (hz + 1) * 6;
-  const long long mxs = (long long)(hy + 1) * (hz + 1) * 6;
-  auto gaddr_tab = [&](int e) {
-    const int mx = e / wz, q = e - mx * wz, kz = kz0 + q;
-    const int mz = kz > hz ? g.Lz - kz : kz;
-    return tab + mx * mxs + mz * 6;
-  };
+  auto stl = [&](int pos, int col, C<T> v) { lds[imul24(pos, t.nc) + col] = v; };
+  const FastDiv dwz = FastDiv::make((unsigned)t.wz);
   C<T> gpre[2][6];
-  {
-    const int ne = (hx + 1) * wz;
-#pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      const int e = threadIdx.x + u * NT;
-      if (e < ne) {
-        const C<T>* gp = gaddr_tab(e);
-#pragma unroll
-        for (int c = 0; c < 6; ++c) gpre[u][c] = gp[c];
-      }
-    }
-  }
+  xtile_prefetch<T, L, NT>(t, gpre, table, table_stride, g, dwz);
   ct_fft<T, L, false, false>(cm, tw, lds, ld0, stl);
-
   __syncthreads();
-  const int nent = (hx + 1) * wz;
-  const FastDiv dwz = FastDiv::make((unsigned)wz);
-  auto apply = [&](int e, const C<T> (&gq)[6]) {
-    const int mx = (int)dwz.div((unsigned)e), q = e - mx * wz, kz = kz0 + q;
-    const bool fz = kz > hz;
-    const C<T> g0 = cscale(gq[0], scale), g1 = cscale(gq[1], scale), g2 = cscale(gq[2], scale);
-    const C<T> g3 = cscale(gq[3], scale), g4 = cscale(gq[4], scale), g5 = cscale(gq[5], scale);
-    const int nkx = (mx == 0 || mx == hx) ? 1 : 2;
-    for (int ix = 0; ix < nkx; ++ix) {
-      const int kx = ix ? L - mx : mx;
-      const bool fx = ix != 0;
-      for (int yr = 0; yr < nyr; ++yr) {
-        const bool fy = yr != 0;
-        const T sxy = (fx != fy) ? T(-1) : T(1), sxz = (fx != fz) ? T(-1) : T(1), syz = (fy != fz) ? T(-1) : T(1);
-        const C<T> h1 = cscale(g1, sxy), h2 = cscale(g2, sxz), h4 = cscale(g4, syz);
-        C<T>* dp = lds + kx * nc + yr * cw + 3 * q;
-        const C<T> x0 = dp[0], x1 = dp[1], x2 = dp[2];
-        dp[0] = cadd(cadd(cmul(g0, x0), cmul(h1, x1)), cmul(h2, x2));
-        dp[1] = cadd(cadd(cmul(h1, x0), cmul(g3, x1)), cmul(h4, x2));
-        dp[2] = cadd(cadd(cmul(h2, x0), cmul(h4, x1)), cmul(g5, x2));
-      }
-    }
-  };
-#pragma unroll
-  for (int u = 0; u < 2; ++u) {
-    const int e = threadIdx.x + u * NT;
-    if (e < nent) apply(e, gpre[u]);
-  }
-  for (int e = threadIdx.x + 2 * NT; e < nent; e += NT) {
-    C<T> gq[6];
-    const C<T>* gp = gaddr_tab(e);
-#pragma unroll
-    for (int c = 0; c < 6; ++c) gq[c] = gp[c];
-    apply(e, gq);
-  }
+  xtile_product<T, L, NT>(t, gpre, table, table_stride, g, lds, cw, scale, dwz);
   __syncthreads();
-  auto ldl = [&](int pos, int col) { return lds[pos * nc + col]; };
+  auto ldl = [&](int pos, int col) { return lds[imul24(pos, t.nc) + col]; };
   auto stN = [&](int pos, int col, C<T> v) {
     bool ok;
-    const long long ad = gaddr(pos, col, ok);
-    if (pos < nx && ok) base[ad] = v;
+    const int ad = xtile_addr(t, pos, col, cw, xs, ok);
+    if (pos < nx && ok) t.base[ad] = v;
   };
   ct_fft<T, L, true, true>(cm, tw, lds, ldl, stN);
 }
@@ -228,14 +275,14 @@ __global__ __launch_bounds__(FFT_NT) void kf_zinv(const C<T>* __restrict__ A, C<
   const int nin = rows * ri;
   for (int e = threadIdx.x; e < nin; e += FFT_NT) {
     const int row = e / ri, r2 = e - row * ri, p = r2 / 3, c = r2 - 3 * p;
-    lds[p * C3 + row * 3 + c] = src[e];
+    lds[imul24(p, C3) + row * 3 + c] = src[e];
   }
   __syncthreads();
   const ColMap cm = ColMap::make(C3);
   const int nz = g.nz;
-  auto ldl = [&](int pos, int col) { return lds[pos * C3 + col]; };
+  auto ldl = [&](int pos, int col) { return lds[imul24(pos, C3) + col]; };
   auto stN = [&](int pos, int col, C<T> v) {
-    if (pos < nz) lds[pos * C3 + col] = v;
+    if (pos < nz) lds[imul24(pos, C3) + col] = v;
   };
   ct_fft<T, L, true, true>(cm, tw, lds, ldl, stN);
   __syncthreads();
@@ -677,6 +724,9 @@ void fast_setup(ConvPlanBase* p) {
   const long long target = 2048;
   for (int i = 0; i < 5; ++i) p->fast[i] = 0;
   if (const char* e = getenv("PDDA_DISABLE_FAST"); e && e[0] == '1') return;
+  // the fast kernels index one slot's A and B with 32-bit offsets built from 24-bit
+  // products (imul24); lengths up to 512 stay far inside both limits
+  if (g.a_stride >= (1LL << 31) || g.b_stride >= (1LL << 31) || 3LL * g.Ly * g.Lz >= (1LL << 24)) return;
   // P1/P5: C = 3 * rows
   if (has_plan(g.Lz)) {
     long long want = std::max(1LL, (long long)g.nx * g.ny * g.nslots / target);

@@ -118,29 +118,19 @@ template <typename T, int L> constexpr int ct_ept() {
   return sizeof(T) == 8 ? (L >= 384 ? 16 : 12) : (L >= 384 ? 32 : 24);
 }
 
-// One Stockham pass: element loads through `ld(pos, col)`, stores through
-// `st(pos, col, v)`. MID_SYNC barriers between the loads and the stores (in-place LDS).
-template <typename T, int L, int R, int NS, bool INV, int EPT, bool MID_SYNC, class LD, class ST>
-__device__ inline void ct_pass(const ColMap& cm, const C<T>* tw, LD ld, ST st) {
+// Compute half of one Stockham pass on the butterflies held in v (batch q is
+// butterfly j = jb + q * jstep of this thread's column): twiddles, radix-R DFT, stores
+// through `st(pos, col, v)`.
+template <typename T, int L, int R, int NS, bool INV, int NB, class ST>
+__device__ inline void ct_pass_regs(const ColMap& cm, const C<T>* tw, C<T> (&v)[NB][R], ST st) {
   constexpr int LR = L / R;
-  constexpr int MAXB = (EPT + R - 1) / R;
   constexpr int TS = L / (NS * R);
   const int t = threadIdx.x;
   const int jb = cm.div(t);
   const int col = t - jb * cm.C;
   const bool active = jb < cm.jstep;
-  C<T> v[MAXB][R];
 #pragma unroll
-  for (int q = 0; q < MAXB; ++q) {
-    const int j = jb + q * cm.jstep;
-    if (active && j < LR) {
-#pragma unroll
-      for (int r = 0; r < R; ++r) v[q][r] = ld(j + r * LR, col);
-    }
-  }
-  if (MID_SYNC) __syncthreads();
-#pragma unroll
-  for (int q = 0; q < MAXB; ++q) {
+  for (int q = 0; q < NB; ++q) {
     const int j = jb + q * cm.jstep;
     if (active && j < LR) {
       const int k = j % NS;
@@ -163,14 +153,37 @@ __device__ inline void ct_pass(const ColMap& cm, const C<T>* tw, LD ld, ST st) {
   }
 }
 
+// One Stockham pass: element loads through `ld(pos, col)`, stores through
+// `st(pos, col, v)`. MID_SYNC barriers between the loads and the stores (in-place LDS).
+template <typename T, int L, int R, int NS, bool INV, int EPT, bool MID_SYNC, class LD, class ST>
+__device__ inline void ct_pass(const ColMap& cm, const C<T>* tw, LD ld, ST st) {
+  constexpr int LR = L / R;
+  constexpr int MAXB = (EPT + R - 1) / R;
+  const int t = threadIdx.x;
+  const int jb = cm.div(t);
+  const int col = t - jb * cm.C;
+  const bool active = jb < cm.jstep;
+  C<T> v[MAXB][R];
+#pragma unroll
+  for (int q = 0; q < MAXB; ++q) {
+    const int j = jb + q * cm.jstep;
+    if (active && j < LR) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[q][r] = ld(j + r * LR, col);
+    }
+  }
+  if (MID_SYNC) __syncthreads();
+  ct_pass_regs<T, L, R, NS, INV, MAXB>(cm, tw, v, st);
+}
+
 // Runs the passes of RList<R...> with Ns = NS. First pass: loads via ld0 (barrier
 // between its loads and stores only if FIRST_SYNC, i.e. ld0 reads the LDS tile);
 // middle passes: LDS in place; last pass: stores via stN. Barriers separate passes.
 template <typename T, int L, bool INV, int EPT, bool FIRST_SYNC, bool FIRST, int NS, class LD0, class STN, int R0,
           int... RR>
 __device__ inline void ct_run(const ColMap& cm, const C<T>* tw, C<T>* lds, LD0 ld0, STN stN) {
-  auto ldl = [&](int pos, int col) { return lds[pos * cm.C + col]; };
-  auto stl = [&](int pos, int col, C<T> v) { lds[pos * cm.C + col] = v; };
+  auto ldl = [&](int pos, int col) { return lds[imul24(pos, cm.C) + col]; };
+  auto stl = [&](int pos, int col, C<T> v) { lds[imul24(pos, cm.C) + col] = v; };
   constexpr bool last = sizeof...(RR) == 0;
   if constexpr (FIRST) {
     if constexpr (last) ct_pass<T, L, R0, NS, INV, EPT, FIRST_SYNC>(cm, tw, ld0, stN);

@@ -222,6 +222,11 @@ __device__ inline void tile_fft(C<T>* data, int ncol, const FftPlan& pl, const C
 }
 
 // Copies the length-L root table into LDS (caller barriers before use).
+// Product of two non-negative indices whose product stays below 2^31 and whose factors
+// are below 2^24: a full-rate 24-bit multiply (the 32-bit multiply issues at a quarter
+// of the VALU rate). Tile and row offsets in the FFT passes are such products.
+__device__ inline int imul24(int a, int b) { return __umul24((unsigned)a, (unsigned)b); }
+
 template <typename T>
 __device__ inline void load_twiddles(C<T>* dst, const C<T>* __restrict__ src, int L) {
   for (int i = threadIdx.x; i < L; i += blockDim.x) dst[i] = src[i];
