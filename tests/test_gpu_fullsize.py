@@ -1,0 +1,126 @@
+"""Two benchmark workloads at their full sizes.
+
+- 64^3 sphere (precision workload: R = 32d, 137376 dipoles, m = 1.33+0.001i, x = 10,
+  FFT grid 128^3), checked against the oracle at the same size:
+  - the interaction table;
+  - the complex128 and complex64 matvecs;
+  - the FP64 true residual of the mixed-precision solve of both polarisations.
+- 256x256x128 porous cuboid (large workload: 5.9M dipoles, FFT grid 512x512x256).
+  It is too large for the oracle, so the test checks size-independent properties of
+  the operator A = diag(1/alpha) - mask G mask on vectors supported on the target:
+  - complex symmetry <y, A x> = <x, A y> (bilinear, no conjugate: the DDA matrix is
+    symmetric, not Hermitian, so a transposed tensor component, a wrong mirror sign
+    or a misplaced kx/ky/kz breaks it);
+  - linearity;
+  - the complex64 product against the complex128 one.
+
+Tolerances: table and complex128 products 1e-13 relative (FFT rounding; measured
+4e-16 and 1e-16), symmetry 1e-14 and linearity 1e-13 relative (sums over 18M
+terms), complex64 products 1e-6 (measured 7e-8 and 9e-8), and the mixed solve's
+true residual below its 1e-6 target. Symmetry is measured as |<y,Ax> - <x,Ay>| /
+(|y| |Ax|). For random vectors an asymmetric operator makes that about N^-1/2, near
+2e-4 here.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import dda_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _site_operands(alpha):
+    """Per-site inverse polarisability and occupancy mask of a workload (device tensors)."""
+    a = alpha.reshape(-1).to(torch.complex128)
+    occ = a != 0
+    ia = torch.where(occ, 1.0 / torch.where(occ, a, torch.ones_like(a)), torch.zeros_like(a))
+    return ia, occ
+
+
+def test_precision_workload_matches_oracle():
+    import precondda_amd as P
+    from precondda_amd import workloads
+    wl = workloads.precision()
+    grid = tuple(int(v) for v in wl.grid)
+    n = int(np.prod(grid))
+    ia_d, occ_d = _site_operands(wl.alpha)
+    ia, mask = ia_d.cpu().numpy(), occ_d.cpu().numpy().astype(np.uint8)
+    print("grid", grid, "dipoles", int(mask.sum()), "k", wl.k, flush=True)
+    assert int(mask.sum()) == 137376
+    G = O.generate_interaction_row(*grid, wl.k, 1.0)
+    G64, G32 = P.generate_interaction_row(*grid, wl.k, 1.0, both=True)
+    terr = np.abs(G64.cpu().numpy() - G).max() / np.abs(G).max()
+    print("table err", terr, flush=True)
+    assert terr < 1e-13
+
+    rng = np.random.default_rng(64)
+    x = rng.standard_normal(3 * n) + 1j * rng.standard_normal(3 * n)
+    ia3 = np.repeat(ia, 3)
+    ref = O.interaction_mult(x, G, ia3, mask, grid)
+    out = P.interaction_mult(x, None, G64, ia, mask, None, *grid).cpu().numpy()
+    err = np.linalg.norm(out - ref) / np.linalg.norm(ref)
+    out32 = P.interaction_mult(x, None, G32, ia, mask, None, *grid).cpu().numpy()
+    err32 = np.linalg.norm(out32 - ref) / np.linalg.norm(ref)
+    print("matvec err complex128", err, "complex64", err32, flush=True)
+    assert err < 1e-13
+    assert err32 < 1e-6
+
+    E = wl.E  # (2, nx, ny, nz, 3): x- and y-polarised plane waves along +z
+    Pm, infos = P.solve_mixed(grid, ia_d, G64, G32, None, E, occ_d, tol=1e-6, inner_target=1e-3, precon=False)
+    Pm = Pm.cpu().numpy().reshape(-1, 3 * n)
+    En = E.cpu().numpy().reshape(-1, *grid, 3)
+    for q in range(En.shape[0]):
+        rel = O.true_residual(Pm[q], En[q], ia, mask.reshape(grid), G, grid)
+        print("rhs", q, infos[q], "true residual", rel, flush=True)
+        assert rel < 1e-6
+
+
+def test_large_workload_operator_properties():
+    import precondda_amd as P
+    from precondda_amd import _lib
+    from precondda_amd.operators import ConvPlan
+    from precondda_amd import workloads
+    wl = workloads.porous_cuboid()
+    grid = tuple(int(v) for v in wl.grid)
+    n = int(np.prod(grid))
+    ia, occ = _site_operands(wl.alpha)
+    mask = occ.to(torch.uint8)
+    print("grid", grid, "dipoles", int(occ.sum()), "k", wl.k, flush=True)
+    G64, G32 = P.generate_interaction_row(*grid, wl.k, 1.0, both=True)
+    g = torch.Generator(device="cuda").manual_seed(5)
+    on = occ.repeat_interleave(3).to(torch.complex128)
+
+    def rand():
+        re = torch.randn(3 * n, generator=g, device="cuda", dtype=torch.float64)
+        im = torch.randn(3 * n, generator=g, device="cuda", dtype=torch.float64)
+        return torch.complex(re, im) * on
+
+    x, y = rand(), rand()
+    plan = ConvPlan(1, grid, grid)
+    plan.prepare_table(G64)
+
+    def A(v):
+        out = torch.empty_like(v)
+        plan.apply(_lib.PDDA_MODE_INTERACT, v.contiguous(), out, G64, inv_alpha=ia, mask=mask)
+        return out
+
+    Ax, Ay = A(x), A(y)
+    yAx, xAy = torch.sum(y * Ax).item(), torch.sum(x * Ay).item()
+    scale = (torch.linalg.vector_norm(y) * torch.linalg.vector_norm(Ax)).item()
+    sym = abs(yAx - xAy) / scale
+    a, b = 0.3 - 1.2j, -0.7 + 0.4j
+    lin = (torch.linalg.vector_norm(A(a * x + b * y) - (a * Ax + b * Ay)) /
+           torch.linalg.vector_norm(a * Ax + b * Ay)).item()
+    print("symmetry", sym, "linearity", lin, flush=True)
+    assert sym < 1e-14
+    assert lin < 1e-13
+
+    plan32 = ConvPlan(0, grid, grid)
+    plan32.prepare_table(G32)
+    x32 = x.to(torch.complex64)
+    out32 = torch.empty_like(x32)
+    plan32.apply(_lib.PDDA_MODE_INTERACT, x32, out32, G32, inv_alpha=ia.to(torch.complex64), mask=mask)
+    err32 = (torch.linalg.vector_norm(out32.to(torch.complex128) - Ax) / torch.linalg.vector_norm(Ax)).item()
+    print("complex64 vs complex128", err32, flush=True)
+    assert err32 < 1e-6
