@@ -247,17 +247,20 @@ def main():
     # HBM traffic of the roofline kernel per launch, from the rocprofv3 FETCH_SIZE /
     # WRITE_SIZE runs of this workload committed under profiles/ (used only when the
     # same kernel was chosen here)
-    try:
-        with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                               "matvec_64cube_traffic.json")) as f:
-            tr = json.load(f)
-        kname = mv["c64"]["kernels"][dom].split(" (")[0]
-        if tuple(tr["grid"]) == tuple(wl.grid) and kname in tr["kernels"] and dom == 2:
-            roofline["traffic"] = tr["kernels"][kname]["traffic_bytes"]
-            roofline["traffic_unit"] = "bytes per launch (rocprofv3 FETCH_SIZE x%.0f + WRITE_SIZE)" % tr["fetch_factor"]
-            roofline["algorithmic_bytes"] = int(byts64[dom])
-    except (OSError, ValueError, KeyError):
-        pass
+    desc = mv["c64"]["kernels"][dom]
+    kname = desc.split(" (")[0] + (" (512 threads)" if "512 threads" in desc else "")
+    for fname in ("matvec_64cube_traffic.json", "matvec_large_traffic.json"):
+        try:
+            with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", fname)) as f:
+                tr = json.load(f)
+            if tuple(tr["grid"]) == tuple(wl.grid) and kname in tr["kernels"] and dom == 2:
+                roofline["traffic"] = tr["kernels"][kname]["traffic_bytes"]
+                roofline["traffic_unit"] = ("bytes per launch (rocprofv3 FETCH_SIZE x%.0f + WRITE_SIZE)" %
+                                            tr["fetch_factor"])
+                roofline["algorithmic_bytes"] = int(byts64[dom])
+                break
+        except (OSError, ValueError, KeyError):
+            pass
 
     # ---------------- CPU baseline (oracle restatement, bounded sample, extrapolated)
     cpu = None
