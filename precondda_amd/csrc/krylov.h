@@ -40,11 +40,15 @@ template <typename T> struct SolverT {
   int sidx = 0;
   long long host_iter = 0;
   bool lockstep = true;  // all slots started together: replacement only at iteration 1
+  // Iterations launched since each slot's last load, saturating at 2. A slot's
+  // true-residual replacement runs in its second iteration (device it == 1), so the
+  // replacement launches are enqueued only in iterations where some slot's age is 1.
+  std::vector<int> age;
   // captured iteration graphs, keyed by what the captured launch sequence depends on
   struct GraphEntry {
     hipGraphExec_t exec = nullptr;
     int n = 0;
-    bool replace = false;
+    int replace = 0;  // bit k: iteration k of the graph carries the replacement launches
     int sidx = 0;
   };
   static constexpr int kMaxGraphs = 8;
@@ -67,6 +71,8 @@ template <typename T> struct SolverT {
   int refresh_tables(int slot, hipStream_t s);
   int load(int slot, const void* E, double target, int max_iter, double ratio, int stagnation, hipStream_t s);
   int iterate(int n_iter, hipStream_t s);
+  int replace_mask(int n) const;
+  void advance_ages(int n);
   // Same iterations replayed from a captured hipGraph (n_iter even; captured on first
   // use and whenever n_iter or the stream changes).
   int iterate_graph(int n_iter, hipStream_t s);

@@ -325,6 +325,7 @@ int SolverT<T>::init(int nx_, int ny_, int nz_, int px_, int py_, int pz_, int n
   PDDA_HIP_CHECK(hipMemset(ctl, 0, nslots * sizeof(SlotCtl)));
   PDDA_HIP_CHECK(hipHostMalloc((void**)&ctl_host, nslots * sizeof(SlotCtl), hipHostMallocDefault));
   std::memset(ctl_host, 0, nslots * sizeof(SlotCtl));
+  age.assign(nslots, 2);
   PDDA_HIP_CHECK(hipMalloc(&partial, (size_t)nslots * nwg * 6 * sizeof(cd)));
   PDDA_HIP_CHECK(hipMalloc(&counters, nslots * sizeof(int)));
   PDDA_HIP_CHECK(hipMemset(counters, 0, nslots * sizeof(int)));
@@ -380,8 +381,24 @@ int SolverT<T>::load(int slot, const void* E, double target, int max_iter, doubl
     sidx = 0;
     host_iter = 0;
   }
+  for (int q = s0; q < s0 + ns; ++q) age[q] = 0;
   PDDA_LAUNCH_CHECK();
   return PDDA_OK;
+}
+
+// Bit k set: some slot reaches its second iteration at the k-th of the next n
+// iterations (only k = 0 and k = 1 can be set).
+template <typename T>
+int SolverT<T>::replace_mask(int n) const {
+  int m = 0;
+  for (int a : age)
+    if (a <= 1 && 1 - a < n) m |= 1 << (1 - a);
+  return m;
+}
+
+template <typename T>
+void SolverT<T>::advance_ages(int n) {
+  for (int& a : age) a = std::min(2, a + n);
 }
 
 template <typename T>
@@ -431,7 +448,7 @@ int SolverT<T>::iterate(int n_iter, hipStream_t s) {
     if ((rc = conv(&mv, MODE_INTERACT, sh, vs.t, ghat, ghat_stride, DOT_ZETA, vs.y, s_new, NEED_ACTIVE, s)))
       return rc;
     hipLaunchKernelGGL(k_upd2<T>, vg, dim3(VNT), 0, s, vs, s_new, s_prev, vs.y, n3, partial, counters, ctl, nwg);
-    const bool replace = !lockstep || host_iter == 1;
+    const bool replace = (replace_mask(1) & 1) != 0;
     hipLaunchKernelGGL(k_vec_finalize<2>, dim3(nslots), dim3(VNT), 0, s, (const cd*)partial, nwg, ctl, 0,
                        (int)FIN_CONTROL, 1, replace ? 1.0 : 0.0, 0.0, 0, 0);
     // true-residual replacement at the slot's iteration 1 (its finalize runs the
@@ -461,6 +478,7 @@ int SolverT<T>::iterate(int n_iter, hipStream_t s) {
     hipLaunchKernelGGL(k_upd3<T>, vg, dim3(VNT), 0, s, vs, n3, ctl, nwg);
     sidx ^= 1;
     ++host_iter;
+    advance_ages(1);
   }
   PDDA_LAUNCH_CHECK();
   return PDDA_OK;
@@ -471,7 +489,7 @@ int SolverT<T>::iterate_graph(int n_iter, hipStream_t s) {
   if (n_iter <= 0) return PDDA_OK;
   if (n_iter % 2) { set_error("iterate_graph: n_iter must be even (s/s_prev ping-pong)"); return PDDA_ERR_ARG; }
   if (lockstep && host_iter < 2) return iterate(n_iter, s);  // replacement step runs eagerly
-  const bool rep = !lockstep;
+  const int rep = replace_mask(n_iter);
   GraphEntry* ge = nullptr;
   for (GraphEntry& g : graphs)
     if (g.n == n_iter && g.replace == rep && g.sidx == sidx) ge = &g;
@@ -486,11 +504,13 @@ int SolverT<T>::iterate_graph(int n_iter, hipStream_t s) {
     hipGraph_t gr = nullptr;
     const long long hi = host_iter;
     const int si = sidx;
+    const std::vector<int> ages = age;
     PDDA_HIP_CHECK(hipStreamBeginCapture(cap_stream, hipStreamCaptureModeRelaxed));
     int rc = iterate(n_iter, cap_stream);
     hipError_t e = hipStreamEndCapture(cap_stream, &gr);
     host_iter = hi;  // capture enqueued nothing
     sidx = si;
+    age = ages;
     if (rc) { if (gr) (void)hipGraphDestroy(gr); return rc; }
     if (e != hipSuccess) { set_error(std::string("graph capture: ") + hipGetErrorString(e)); return PDDA_ERR_HIP; }
     GraphEntry g;
@@ -505,6 +525,7 @@ int SolverT<T>::iterate_graph(int n_iter, hipStream_t s) {
   }
   PDDA_HIP_CHECK(hipGraphLaunch(ge->exec, s));
   host_iter += n_iter;
+  advance_ages(n_iter);
   return PDDA_OK;
 }
 
