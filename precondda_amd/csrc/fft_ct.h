@@ -118,19 +118,29 @@ template <typename T, int L> constexpr int ct_ept() {
   return sizeof(T) == 8 ? (L >= 384 ? 16 : 12) : (L >= 384 ? 32 : 24);
 }
 
-// Compute half of one Stockham pass on the butterflies held in v (batch q is
-// butterfly j = jb + q * jstep of this thread's column): twiddles, radix-R DFT, stores
-// through `st(pos, col, v)`.
-template <typename T, int L, int R, int NS, bool INV, int NB, class ST>
-__device__ inline void ct_pass_regs(const ColMap& cm, const C<T>* tw, C<T> (&v)[NB][R], ST st) {
+// One Stockham pass: element loads through `ld(pos, col)`, stores through
+// `st(pos, col, v)`. MID_SYNC barriers between the loads and the stores (in-place LDS).
+template <typename T, int L, int R, int NS, bool INV, int EPT, bool MID_SYNC, class LD, class ST>
+__device__ inline void ct_pass(const ColMap& cm, const C<T>* tw, LD ld, ST st) {
   constexpr int LR = L / R;
+  constexpr int MAXB = (EPT + R - 1) / R;
   constexpr int TS = L / (NS * R);
   const int t = threadIdx.x;
   const int jb = cm.div(t);
   const int col = t - jb * cm.C;
   const bool active = jb < cm.jstep;
+  C<T> v[MAXB][R];
 #pragma unroll
-  for (int q = 0; q < NB; ++q) {
+  for (int q = 0; q < MAXB; ++q) {
+    const int j = jb + q * cm.jstep;
+    if (active && j < LR) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[q][r] = ld(j + r * LR, col);
+    }
+  }
+  if (MID_SYNC) __syncthreads();
+#pragma unroll
+  for (int q = 0; q < MAXB; ++q) {
     const int j = jb + q * cm.jstep;
     if (active && j < LR) {
       const int k = j % NS;
@@ -151,29 +161,6 @@ __device__ inline void ct_pass_regs(const ColMap& cm, const C<T>* tw, C<T> (&v)[
       for (int r = 0; r < R; ++r) st(base + r * NS, col, INV ? cconj(v[q][r]) : v[q][r]);
     }
   }
-}
-
-// One Stockham pass: element loads through `ld(pos, col)`, stores through
-// `st(pos, col, v)`. MID_SYNC barriers between the loads and the stores (in-place LDS).
-template <typename T, int L, int R, int NS, bool INV, int EPT, bool MID_SYNC, class LD, class ST>
-__device__ inline void ct_pass(const ColMap& cm, const C<T>* tw, LD ld, ST st) {
-  constexpr int LR = L / R;
-  constexpr int MAXB = (EPT + R - 1) / R;
-  const int t = threadIdx.x;
-  const int jb = cm.div(t);
-  const int col = t - jb * cm.C;
-  const bool active = jb < cm.jstep;
-  C<T> v[MAXB][R];
-#pragma unroll
-  for (int q = 0; q < MAXB; ++q) {
-    const int j = jb + q * cm.jstep;
-    if (active && j < LR) {
-#pragma unroll
-      for (int r = 0; r < R; ++r) v[q][r] = ld(j + r * LR, col);
-    }
-  }
-  if (MID_SYNC) __syncthreads();
-  ct_pass_regs<T, L, R, NS, INV, MAXB>(cm, tw, v, st);
 }
 
 // Runs the passes of RList<R...> with Ns = NS. First pass: loads via ld0 (barrier
