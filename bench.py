@@ -46,7 +46,9 @@ def parse():
                     help="serving: Poisson arrival rate (requests/s); default 0.8x the measured saturated throughput")
     ap.add_argument("--requests", type=int, default=200, help="serving: number of requests")
     ap.add_argument("--slots", type=int, default=16, help="serving: slots per padded-box bin")
-    ap.add_argument("--check-every", type=int, default=16, help="serving: most inner iterations per launch")
+    ap.add_argument("--check-every", type=int, default=None,
+                    help="inner iterations per launch between host status checks (even; default 8, "
+                         "serving: most per launch, default 16)")
     ap.add_argument("--max-active", type=int, default=0,
                     help="serving: bins in flight at once, oldest request first (0: every bin)")
     ap.add_argument("--share", default="bin", choices=["bin", "slot"],
@@ -133,7 +135,7 @@ def main():
 
     def mixed_step():
         ms.solve(E64, tol=args.tol, inner_target=args.inner_target, max_outer=12, max_inner=3000, ratio=100.0,
-                 check_every=8)
+                 check_every=args.check_every or 8)
         return ms.result()
 
     for w in range(args.warmup):
@@ -347,7 +349,7 @@ def serving_main(args):
 
     def engine():
         return ServingEngine(hbm_budget=args.hbm_budget, slots_per_bin=args.slots, tol=args.tol,
-                             check_every=args.check_every, quantum_ms=args.quantum_ms or None, share=args.share,
+                             check_every=args.check_every or 16, quantum_ms=args.quantum_ms or None, share=args.share,
                              max_active=args.max_active)
 
     def summary(res, wall):
