@@ -425,6 +425,7 @@ int SolverT<T>::conv(ConvPlanBase* plan, int mode, const C<T>* x, C<T>* out, con
 
 template <typename T>
 int SolverT<T>::iterate(int n_iter, hipStream_t s) {
+  if (!mask) { set_error("solver: operands not bound"); return PDDA_ERR_STATE; }
   int rc;
   const dim3 vg(nwg, nslots);
   for (int k = 0; k < n_iter; ++k) {

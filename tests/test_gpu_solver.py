@@ -90,3 +90,26 @@ def test_mixed_precision_small(monkeypatch, p3):
     ce, _ = O.cross_sections(ref[1], pb["E_inc"], ia, k)
     me, _ = O.cross_sections(Pm, pb["E_inc"], ia, k)
     assert abs(me - ce) < 1e-5 * abs(ce)
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_iterate_before_bind_is_an_error(graph):
+    """Iterating a solver whose operands were never bound returns PDDA_ERR_STATE from
+    the host check, before any launch (it would otherwise read null tables)."""
+    import ctypes
+    from precondda_amd import _lib
+    L = _lib.lib()
+    h = ctypes.c_void_p()
+    assert L.pdda_solver_create(ctypes.byref(h), 1, 8, 8, 8, 8, 8, 8, 2, 0) == 0
+    try:
+        it = L.pdda_solver_iterate_graph if graph else L.pdda_solver_iterate
+        assert it(h, 2, None) == -5  # PDDA_ERR_STATE
+        assert b"not bound" in L.pdda_last_error()
+    finally:
+        L.pdda_solver_destroy(h)
+    m = ctypes.c_void_p()
+    assert L.pdda_mixed_create(ctypes.byref(m), 8, 8, 8, 8, 8, 8, 2, 0) == 0
+    try:
+        assert L.pdda_mixed_iterate(m, 2, 1 if graph else 0, None) == -5
+    finally:
+        L.pdda_mixed_destroy(m)
