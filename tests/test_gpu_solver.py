@@ -94,8 +94,8 @@ def test_mixed_precision_small(monkeypatch, p3):
 
 @pytest.mark.parametrize("graph", [False, True])
 def test_iterate_before_bind_is_an_error(graph):
-    """Iterating a solver whose operands were never bound returns PDDA_ERR_STATE from
-    the host check, before any launch (it would otherwise read null tables)."""
+    """Iterating a solver whose operands were never bound returns PDDA_ERR_STATE with
+    the same message as loading one, from a host check before any launch."""
     import ctypes
     from precondda_amd import _lib
     L = _lib.lib()
