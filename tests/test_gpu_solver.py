@@ -113,3 +113,35 @@ def test_iterate_before_bind_is_an_error(graph):
         assert L.pdda_mixed_iterate(m, 2, 1 if graph else 0, None) == -5
     finally:
         L.pdda_mixed_destroy(m)
+
+
+def test_solver_failure_codes_match_oracle():
+    """The reference's return conventions (Gpbicgstab.py:1173-1201), on the device and in
+    the oracle with the same inputs: an exhausted iteration limit returns max_iter; a
+    NaN in E_inc returns -(i+1) at the first iteration; a one-iteration stagnation
+    window returns -(i+1) at the first step that does not lower the residual. That step
+    is taken on a target (x = 3, no preconditioner) whose residual first rises at
+    iteration 6, far above rounding; near the rounding floor the first rise moves with
+    the last bits."""
+    import precondda_amd as P
+    pb, grid, k, ia, mask, G, M = _setup(3, 1.5 + 0.1j, 1.0)
+    Gd, Md = torch.as_tensor(G).cuda(), torch.as_tensor(M).cuda()
+
+    def both(E, ratio, max_iter, target, **kw):
+        ref = O.solve_gpbicgstab(grid, ia, G, M, E, mask, ratio, max_iter, target, **kw)
+        out = P.solve_gpbicgstab(grid, ia, Gd, Md, E, mask, ratio, max_iter, target, **kw)
+        print("oracle", ref[0], "device", out[0], flush=True)
+        return ref, out
+
+    ref, out = both(pb["E_inc"], 100, 2, 1e-12)
+    assert ref[0] == 2 and out[0] == 2
+    assert abs(out[2] - ref[2]) <= 1e-10 * abs(ref[2])
+    E2 = pb["E_inc"].copy()
+    E2[tuple(np.argwhere(mask)[0]) + (0,)] = np.nan
+    ref, out = both(E2, 100, 50, 1e-12)
+    assert ref[0] == -1 and out[0] == -1
+    pb, grid, k, ia, mask, G, M = _setup(3, 1.5 + 0.1j, 3.0)
+    Gd, Md = torch.as_tensor(G).cuda(), torch.as_tensor(M).cuda()
+    ref, out = both(pb["E_inc"], 0, 200, 0.0, precon=False, stagnation=1)
+    assert ref[0] == -7 and out[0] == -7
+    assert abs(out[2] - ref[2]) <= 1e-10 * abs(ref[2])
