@@ -145,3 +145,25 @@ def test_solver_failure_codes_match_oracle():
     ref, out = both(pb["E_inc"], 0, 200, 0.0, precon=False, stagnation=1)
     assert ref[0] == -7 and out[0] == -7
     assert abs(out[2] - ref[2]) <= 1e-10 * abs(ref[2])
+
+
+def test_run_solver_matches_oracle_and_wraps_failures():
+    """run_solver (ddaforgit.py:1175-1210): ratio forced to 100 and the default target
+    1e-5, preconditioned, FP64 path; same iterations and P as the oracle's run_solver,
+    and a failure inside surfaces as RuntimeError("Solver failed: ...")."""
+    import precondda_amd as P
+    pb, _ = O.sphere_problem(4, 1.5 + 0.01j, 1.0)
+    grid, k = pb["grid_size"], pb["k"]
+    nx, ny, nz = grid
+    rm = 1 - 2j
+    ref = O.run_solver(grid, k, pb["alpha_array"], 1.0, refract_mult=rm, E_inc=pb["E_inc"], precon=True,
+                       max_iter=500, expansion=(nx, ny, nz))
+    out = P.run_solver(grid, k, pb["alpha_array"], 1.0, nx, ny, nz, rm, pb["E_inc"], precon=True, max_iter=500,
+                       double=True, ratio=7)
+    print("iters", ref[0], out[0])
+    assert out[0] == ref[0] and out[0] > 0
+    err = np.linalg.norm(out[1].cpu().numpy() - ref[1]) / np.linalg.norm(ref[1])
+    print("P rel err", err)
+    assert err < 1e-10
+    with pytest.raises(RuntimeError, match="^Solver failed: "):
+        P.run_solver(grid, k, pb["alpha_array"], 1.0, nx, ny, nz, rm, pb["E_inc"].reshape(-1)[:-3], double=True)
