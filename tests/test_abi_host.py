@@ -90,3 +90,11 @@ def test_batch_directions_are_orthonormal_triads():
     assert np.abs(np.sum(p[0::2] * p[1::2], axis=1)).max() < 1e-12
     v = np.random.default_rng(0).standard_normal((64, 3))
     assert np.allclose(d[0::2], v / np.linalg.norm(v, axis=1, keepdims=True))
+
+
+def test_excited_field_size_mismatch_is_value_error():
+    """E_field_solver rejects a polarisation of the wrong size with ValueError and the
+    reference's message (ExcitedField.py:141-152), before any device work."""
+    from precondda_amd import operators
+    with pytest.raises(ValueError, match="^Polarization size 10 doesn't match expected size 24$"):
+        operators.E_field_solver(np.zeros(10, complex), None, (2, 2, 2))
