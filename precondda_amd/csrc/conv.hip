@@ -783,26 +783,36 @@ static int conv_apply_t(ConvPlanBase* p, const ConvArgs& a, hipStream_t s, hipEv
   return PDDA_OK;
 }
 
+// The operands each mode and fused reduction reads (checked before any launch).
+static int check_conv_args(const ConvArgs& a, const char* who) {
+  auto fail = [&](const char* what) {
+    set_error(std::string(who) + ": " + what);
+    return PDDA_ERR_ARG;
+  };
+  if (a.mode < MODE_FIELD || a.mode > MODE_RESIDUAL) return fail("unknown mode");
+  if (!a.x || !a.out || !a.table) return fail("null x/out/table");
+  if (a.mode != MODE_FIELD && !a.mask) return fail("mask required");
+  if ((a.mode == MODE_INTERACT || a.mode == MODE_RESIDUAL) && !a.inv_alpha) return fail("inv_alpha required");
+  if (a.mode == MODE_RESIDUAL && !a.b) return fail("b required");
+  if (a.dots != DOT_NONE && (!a.ctl || !a.d0 || (a.dots == DOT_ZETA && !a.d1)))
+    return fail("fused dots need ctl and operand vectors");
+  return PDDA_OK;
+}
+
 int conv_apply(ConvPlanBase* p, const ConvArgs& a, hipStream_t s) {
-  if (!a.x || !a.out || !a.table) { set_error("conv_apply: null x/out/table"); return PDDA_ERR_ARG; }
-  if (a.mode != MODE_FIELD && !a.mask) { set_error("conv_apply: mask required"); return PDDA_ERR_ARG; }
-  if ((a.mode == MODE_INTERACT || a.mode == MODE_RESIDUAL) && !a.inv_alpha) {
-    set_error("conv_apply: inv_alpha required"); return PDDA_ERR_ARG;
-  }
-  if (a.mode == MODE_RESIDUAL && !a.b) { set_error("conv_apply: b required"); return PDDA_ERR_ARG; }
-  if (a.dots != DOT_NONE && (!a.ctl || !a.d0 || (a.dots == DOT_ZETA && !a.d1))) {
-    set_error("conv_apply: fused dots need ctl and operand vectors"); return PDDA_ERR_ARG;
-  }
+  if (int rc = check_conv_args(a, "conv_apply")) return rc;
   const ConvArgs b = with_planar(p, a);
   return p->prec ? conv_apply_t<double>(p, b, s) : conv_apply_t<float>(p, b, s);
 }
 
 int conv_time_passes(ConvPlanBase* p, const ConvArgs& a, int reps, double* ms5, hipStream_t s) {
   if (reps <= 0 || !ms5) { set_error("conv_time_passes: bad arguments"); return PDDA_ERR_ARG; }
-  hipEvent_t ev[6];
-  for (int i = 0; i < 6; ++i) PDDA_HIP_CHECK(hipEventCreate(&ev[i]));
-  for (int q = 0; q < 5; ++q) ms5[q] = 0.0;
+  if (int rc = check_conv_args(a, "conv_time_passes")) return rc;
+  hipEvent_t ev[6] = {};
   int rc = PDDA_OK;
+  for (int i = 0; i < 6 && rc == PDDA_OK; ++i)
+    if (hipEventCreate(&ev[i]) != hipSuccess) { set_error("conv_time_passes: event create failed"); rc = PDDA_ERR_HIP; }
+  for (int q = 0; q < 5; ++q) ms5[q] = 0.0;
   const ConvArgs ap = with_planar(p, a);
   for (int r = 0; r < reps && rc == PDDA_OK; ++r) {
     rc = p->prec ? conv_apply_t<double>(p, ap, s, ev) : conv_apply_t<float>(p, ap, s, ev);
@@ -815,7 +825,8 @@ int conv_time_passes(ConvPlanBase* p, const ConvArgs& a, int reps, double* ms5, 
     }
   }
   for (int q = 0; q < 5; ++q) ms5[q] /= reps;
-  for (int i = 0; i < 6; ++i) (void)hipEventDestroy(ev[i]);
+  for (int i = 0; i < 6; ++i)
+    if (ev[i]) (void)hipEventDestroy(ev[i]);
   return rc;
 }
 

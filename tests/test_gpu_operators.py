@@ -130,3 +130,34 @@ def test_cross_sections_match_oracle():
     de, da = P.cross_sections(torch.as_tensor(Pol).cuda(), pb["E_inc"], ia, k)
     assert abs(de - ce) < 1e-12 * abs(ce)
     assert abs(da - ca) < 1e-12 * abs(ca)
+
+
+def test_conv_entry_points_reject_missing_operands():
+    """pdda_conv_apply and pdda_conv_time_passes check every operand their mode reads
+    before any launch: an unknown mode, a null table, a residual without b."""
+    import ctypes
+    from precondda_amd import _lib
+    L = _lib.lib()
+    h = ctypes.c_void_p()
+    assert L.pdda_conv_create(ctypes.byref(h), 1, 8, 8, 8, 8, 8, 8, 1) == 0
+    try:
+        x = torch.zeros(3 * 512, dtype=torch.complex128, device="cuda")
+        out = torch.empty_like(x)
+        tab = torch.zeros(9 * 9 * 9 * 6, dtype=torch.complex128, device="cuda")
+        ia = torch.ones(512, dtype=torch.complex128, device="cuda")
+        mask = torch.ones(512, dtype=torch.uint8, device="cuda")
+        p = lambda t: ctypes.c_void_p(t.data_ptr())
+        ms5 = (ctypes.c_double * 5)()
+        assert L.pdda_conv_apply(h, 9, p(x), p(out), p(tab), 0, p(ia), p(mask), 0, None, None) == -1
+        assert b"unknown mode" in L.pdda_last_error()
+        assert L.pdda_conv_apply(h, _lib.PDDA_MODE_RESIDUAL, p(x), p(out), p(tab), 0, p(ia), p(mask), 0, None,
+                                 None) == -1
+        assert b"b required" in L.pdda_last_error()
+        assert L.pdda_conv_time_passes(h, _lib.PDDA_MODE_INTERACT, p(x), p(out), None, p(ia), p(mask), 2, ms5,
+                                       None) == -1
+        assert b"null x/out/table" in L.pdda_last_error()
+        assert L.pdda_conv_time_passes(h, _lib.PDDA_MODE_INTERACT, p(x), p(out), p(tab), p(ia), p(mask), 2, ms5,
+                                       None) == 0
+        torch.cuda.synchronize()
+    finally:
+        L.pdda_conv_destroy(h)
