@@ -251,7 +251,7 @@ def main():
     # same kernel was chosen here)
     desc = mv["c64"]["kernels"][dom]
     kname = desc.split(" (")[0] + (" (512 threads)" if "512 threads" in desc else "")
-    for fname in ("matvec_64cube_traffic.json", "matvec_large_traffic.json"):
+    for fname in ("matvec_64cube_traffic.json", "matvec_batching_traffic.json", "matvec_large_traffic.json"):
         try:
             with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", fname)) as f:
                 tr = json.load(f)
