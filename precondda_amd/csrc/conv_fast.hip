@@ -416,15 +416,15 @@ template <typename T, int L, bool INV> void launch_axis(ConvPlanBase* p, const C
                      ax, (const C<T>*)p->tw[1], a.ctl, a.need);
 }
 
+// Device properties read once per process (thread-safe static initialisation).
 int device_cu_count() {
-  static int n = 0;
-  if (!n) {
+  static const int n = [] {
     int dev = 0, v = 0;
-    n = (hipGetDevice(&dev) == hipSuccess &&
-         hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
-            ? v
-            : 256;
-  }
+    return (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
+               ? v
+               : 256;
+  }();
   return n;
 }
 
@@ -623,15 +623,13 @@ void set_z_tiles(ConvPlanBase* p, int zr, int rows) {
 }
 
 size_t device_lds_limit() {
-  static size_t lim = 0;
-  if (!lim) {
+  static const size_t lim = [] {
     int dev = 0, v = 0;
-    if (hipGetDevice(&dev) == hipSuccess &&
-        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
-      lim = (size_t)v;
-    else
-      lim = 64 * 1024;
-  }
+    return (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
+               ? (size_t)v
+               : (size_t)64 * 1024;
+  }();
   return lim;
 }
 
