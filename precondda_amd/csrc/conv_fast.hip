@@ -790,8 +790,9 @@ std::vector<int> pow2_candidates(int L, int prec, int c3, int pmax, int nt = FFT
   return out;
 }
 
-// Milliseconds per launch sequence of the given passes: the best of three trials of
-// `reps` launches each (candidates differ by a few percent; one trial is too noisy).
+// Milliseconds per launch sequence of the given passes: the best of kTrials trials of
+// `reps` launches each (candidates differ by a few percent; with three trials the
+// choice between close candidates still flipped from one process to the next).
 // With a flush buffer, every sequence starts after `flush_bytes` of writes have pushed
 // the plan's data out of the last-level cache, as in a solve, where the Krylov vectors
 // touched between two matvecs do the same: a kernel that wins on cache-resident data
@@ -800,7 +801,8 @@ float time_passes(ConvPlanBase* p, const int* passes, int np, const ConvArgs& a,
                   hipEvent_t e1, int reps, void* flush = nullptr, size_t flush_bytes = 0) {
   for (int q = 0; q < np; ++q) fast_launch(p, passes[q], a, s);  // warm-up
   float best = 1e30f;
-  for (int trial = 0; trial < 3; ++trial) {
+  constexpr int kTrials = 7;
+  for (int trial = 0; trial < kTrials; ++trial) {
     float total = 0.f;
     if (flush) {
       for (int r = 0; r < reps; ++r) {
