@@ -673,6 +673,13 @@ bool fxv_of_plan(const ConvPlanBase* p) {
   return false;
 }
 
+// PDDA_AUTOTUNE_TRIALS=<n> (1..15, default 3): timed trials per autotuner candidate.
+int autotune_trials() {
+  const char* e = getenv("PDDA_AUTOTUNE_TRIALS");
+  const int n = e ? atoi(e) : 3;
+  return n < 1 ? 1 : (n > 15 ? 15 : n);
+}
+
 // PDDA_AUTOTUNE_LOG=1 prints the timed candidates.
 bool autotune_log() {
   const char* e = getenv("PDDA_AUTOTUNE_LOG");
@@ -790,9 +797,8 @@ std::vector<int> pow2_candidates(int L, int prec, int c3, int pmax, int nt = FFT
   return out;
 }
 
-// Milliseconds per launch sequence of the given passes: the best of kTrials trials of
-// `reps` launches each (candidates differ by a few percent; with three trials the
-// choice between close candidates still flipped from one process to the next).
+// Milliseconds per launch sequence of the given passes: the best of autotune_trials() trials of
+// `reps` launches each (candidates differ by a few percent; one trial is too noisy).
 // With a flush buffer, every sequence starts after `flush_bytes` of writes have pushed
 // the plan's data out of the last-level cache, as in a solve, where the Krylov vectors
 // touched between two matvecs do the same: a kernel that wins on cache-resident data
@@ -801,8 +807,8 @@ float time_passes(ConvPlanBase* p, const int* passes, int np, const ConvArgs& a,
                   hipEvent_t e1, int reps, void* flush = nullptr, size_t flush_bytes = 0) {
   for (int q = 0; q < np; ++q) fast_launch(p, passes[q], a, s);  // warm-up
   float best = 1e30f;
-  constexpr int kTrials = 7;
-  for (int trial = 0; trial < kTrials; ++trial) {
+  const int ntrials = autotune_trials();
+  for (int trial = 0; trial < ntrials; ++trial) {
     float total = 0.f;
     if (flush) {
       for (int r = 0; r < reps; ++r) {
