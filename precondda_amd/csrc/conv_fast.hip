@@ -680,6 +680,13 @@ int autotune_trials() {
   return n < 1 ? 1 : (n > 15 ? 15 : n);
 }
 
+// PDDA_AUTOTUNE_FLUSH_MB=<n> (default 512): bytes written before each timed sequence.
+int autotune_flush_mb() {
+  const char* e = getenv("PDDA_AUTOTUNE_FLUSH_MB");
+  const int n = e ? atoi(e) : 512;
+  return n < 0 ? 0 : (n > 4096 ? 4096 : n);
+}
+
 // PDDA_AUTOTUNE_LOG=1 prints the timed candidates.
 bool autotune_log() {
   const char* e = getenv("PDDA_AUTOTUNE_LOG");
@@ -868,9 +875,10 @@ int fast_autotune(ConvPlanBase* p) {
                                     (size_t)planar_elems(g.Lx / 2, g.Ly / 2, g.Lz / 2));
   PDDA_HIP_CHECK(hipMalloc(&res.tab, tab_elems * es));
   void* tab = res.tab;
-  // twice the 256 MiB last-level cache; timing runs without it when it does not fit
-  const size_t flush_bytes = (size_t)512 << 20;
-  if (hipMalloc(&res.flush, flush_bytes) != hipSuccess) {
+  // twice the 256 MiB last-level cache (PDDA_AUTOTUNE_FLUSH_MB changes it, 0 = none);
+  // timing runs without it when it does not fit
+  const size_t flush_bytes = (size_t)autotune_flush_mb() << 20;
+  if (flush_bytes > 0 && hipMalloc(&res.flush, flush_bytes) != hipSuccess) {
     (void)hipGetLastError();
     res.flush = nullptr;
   }
