@@ -379,7 +379,7 @@ def build_preconditioner(grid, k, d, inv_alpha, refract_mult=1.0, expansion=None
 
     The reference calls pre_preconditioner / circulant_approximation
     (ddaforgit.py:1118-1136) whose source is not in the reference repository, and
-    applies the result with fft_matvec3 (Gpbicgstab.py:633-668). This build defines
+    applies the result with fft_matvec3 (Gpbicgstab.py:633-668). This module defines
     M_hat(q) = (gamma I - Ghat_P(q))^-1 per frequency, where Ghat_P is the reduced FFT of
     -A on the 2P grid and gamma = refract_mult * mean(inv_alpha over occupied sites):
     the exact inverse of the fully occupied periodic operator.
