@@ -65,6 +65,9 @@ struct ConvPlanBase {
   int f_xv = 0;     // P3 register-product kernel: columns per tile (0: LDS Stockham P3)
   int f_zr = 0;     // register z kernels (whole rows, one LDS exchange): bit 0 P1, bit 1 P5
   int f_wide = 0;   // 512-thread tiles (lengths >= 384): bit 0 P2/P4, bit 1 P3 Stockham
+  // Set before conv_plan_init by solvers whose P1 carries the first GPBiCG update
+  // (ConvArgs::upd1): the autotuner then keeps the register P1 where one exists.
+  int p1_upd = 0;
   size_t f_lds[5] = {0, 0, 0, 0, 0};
   // Planar plans time the register P3 on x-planar table copies (ConvArgs::table_x),
   // and keep such a copy of their prepared table (conv_prepare_table).
@@ -103,12 +106,26 @@ struct ConvArgs {
   SlotCtl* ctl = nullptr;
   int need = NEED_ALL;
   bool fin_control = false;  // DOT_RESID finalize also runs the slot control step
+  // First GPBiCG update fused into P1 (register z kernel only, conv_p1_fuses_upd1):
+  // for each slot that `need` selects, P1 computes s = r - alpha v and
+  // y = s_prev - s - alpha w (alpha = ctl[slot].alpha), stores both, and transforms s
+  // instead of reading x. Null r: no update.
+  struct Upd1 {
+    const void* r = nullptr;
+    const void* v = nullptr;
+    const void* sp = nullptr;
+    const void* w = nullptr;
+    void* s = nullptr;
+    void* y = nullptr;
+  } upd1;
 };
 
 // Fast path (conv_fast.hip): chooses compile-time-planned kernels and their tiles for
 // the passes whose FFT length has a compile-time plan; launches one pass.
 void fast_setup(ConvPlanBase* p);
 bool fast_p3_usable(const ConvPlanBase* p, const ConvArgs& a);
+// True when the plan's P1 is the register z kernel, which can take ConvArgs::upd1.
+bool conv_p1_fuses_upd1(const ConvPlanBase* p);
 int fast_launch(ConvPlanBase* p, int pass, const ConvArgs& a, hipStream_t s);
 // Times the candidate tile shapes of every fast pass on the device (on a private
 // stream, with plan-owned scratch) and keeps the fastest. PDDA_NO_AUTOTUNE=1 skips it.

@@ -784,7 +784,7 @@ static int conv_apply_t(ConvPlanBase* p, const ConvArgs& a, hipStream_t s, hipEv
 }
 
 // The operands each mode and fused reduction reads (checked before any launch).
-static int check_conv_args(const ConvArgs& a, const char* who) {
+static int check_conv_args(const ConvPlanBase* p, const ConvArgs& a, const char* who) {
   auto fail = [&](const char* what) {
     set_error(std::string(who) + ": " + what);
     return PDDA_ERR_ARG;
@@ -796,18 +796,24 @@ static int check_conv_args(const ConvArgs& a, const char* who) {
   if (a.mode == MODE_RESIDUAL && !a.b) return fail("b required");
   if (a.dots != DOT_NONE && (!a.ctl || !a.d0 || (a.dots == DOT_ZETA && !a.d1)))
     return fail("fused dots need ctl and operand vectors");
+  if (a.upd1.r) {
+    if (!a.upd1.v || !a.upd1.sp || !a.upd1.w || !a.upd1.s || !a.upd1.y || !a.ctl)
+      return fail("fused update needs r, v, s_prev, w, s, y and ctl");
+    if (a.upd1.s != a.x) return fail("fused update: x must be the s it writes (the epilogue reads x)");
+    if (!conv_p1_fuses_upd1(p)) return fail("fused update needs the register z-forward pass");
+  }
   return PDDA_OK;
 }
 
 int conv_apply(ConvPlanBase* p, const ConvArgs& a, hipStream_t s) {
-  if (int rc = check_conv_args(a, "conv_apply")) return rc;
+  if (int rc = check_conv_args(p, a, "conv_apply")) return rc;
   const ConvArgs b = with_planar(p, a);
   return p->prec ? conv_apply_t<double>(p, b, s) : conv_apply_t<float>(p, b, s);
 }
 
 int conv_time_passes(ConvPlanBase* p, const ConvArgs& a, int reps, double* ms5, hipStream_t s) {
   if (reps <= 0 || !ms5) { set_error("conv_time_passes: bad arguments"); return PDDA_ERR_ARG; }
-  if (int rc = check_conv_args(a, "conv_time_passes")) return rc;
+  if (int rc = check_conv_args(p, a, "conv_time_passes")) return rc;
   hipEvent_t ev[6] = {};
   int rc = PDDA_OK;
   for (int i = 0; i < 6 && rc == PDDA_OK; ++i)

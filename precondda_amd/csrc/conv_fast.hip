@@ -381,9 +381,16 @@ template <typename T, int L> void launch_p1(ConvPlanBase* p, const ConvArgs& a, 
   if constexpr (XPlan<L>::ok) {
     if (p->f_zr & 1) {
       const long long tiles = ((long long)g.nx * g.ny + ZTile<L>::R - 1) / ZTile<L>::R;
-      hipLaunchKernelGGL((kz_zfwd<T, L>), dim3((unsigned)(g.nslots * tiles)), dim3(ZTile<L>::NT),
-                         (size_t)ZTile<L>::lds_elems() * sizeof(C<T>), s, (const C<T>*)a.x, (C<T>*)p->work_a, g,
-                         (const C<T>*)p->tw[2], a.ctl, a.need);
+      const ZUpd<T> u{(const C<T>*)a.upd1.r, (const C<T>*)a.upd1.v, (const C<T>*)a.upd1.sp,
+                      (const C<T>*)a.upd1.w, (C<T>*)a.upd1.s, (C<T>*)a.upd1.y};
+      if (a.upd1.r)
+        hipLaunchKernelGGL((kz_zfwd<T, L, true>), dim3((unsigned)(g.nslots * tiles)), dim3(ZTile<L>::NT),
+                           (size_t)ZTile<L>::lds_elems() * sizeof(C<T>), s, (const C<T>*)a.x, (C<T>*)p->work_a, g,
+                           (const C<T>*)p->tw[2], a.ctl, a.need, u);
+      else
+        hipLaunchKernelGGL((kz_zfwd<T, L>), dim3((unsigned)(g.nslots * tiles)), dim3(ZTile<L>::NT),
+                           (size_t)ZTile<L>::lds_elems() * sizeof(C<T>), s, (const C<T>*)a.x, (C<T>*)p->work_a, g,
+                           (const C<T>*)p->tw[2], a.ctl, a.need, u);
       return;
     }
   }
@@ -922,8 +929,14 @@ int fast_autotune(ConvPlanBase* p) {
       }
     }
     if (zforce != 1 && zreg_rows(g.Lz) > 0) {
-      // register P1, P5 or both, beside the best LDS row tile
+      // register P1, P5 or both, beside the best LDS row tile. A plan whose P1 carries
+      // the solver's first vector update keeps the register P1 (mask 1 or 3): that
+      // saves the update's own launch and one vector read per iteration, more than
+      // the P1 kernels differ by.
+      const bool keep_p1 = p->p1_upd && zforce == 0;
+      if (keep_p1) best = 1e30f;
       for (int zr = (zforce == 2 ? 3 : 1); zr <= 3; ++zr) {
+        if (keep_p1 && !(zr & 1)) continue;
         set_z_tiles(p, zr, bestr);
         const float t = time_passes(p, passes, np, az, s, e0, e1, reps, fl, flush_bytes);
         if (autotune_log()) fprintf(stderr, "[pdda autotune] P1+P5 register mask %d: %.2f us\n", zr, 1e3 * t);
@@ -1007,6 +1020,10 @@ int fast_autotune(ConvPlanBase* p) {
   hipError_t err = hipStreamSynchronize(s);
   if (err != hipSuccess) { set_error(std::string("autotune: ") + hipGetErrorString(err)); return PDDA_ERR_HIP; }
   return PDDA_OK;
+}
+
+bool conv_p1_fuses_upd1(const ConvPlanBase* p) {
+  return p->fast[0] && (p->f_zr & 1) && zreg_rows(p->g.Lz) > 0;
 }
 
 bool fast_p3_usable(const ConvPlanBase* p, const ConvArgs& a) {

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -96,9 +97,6 @@ __global__ __launch_bounds__(VNT) void k_vec_finalize(const cd* __restrict__ par
 }
 
 template <typename T> __device__ inline C<T> cz() { return mk<T>(0, 0); }
-template <typename T> __device__ inline C<T> smul(cd s, C<T> a) {
-  return mk<T>((T)s.x * a.x - (T)s.y * a.y, (T)s.x * a.y + (T)s.y * a.x);
-}
 
 // b = E * mask; r = rhat = p = b; x = best = w = u = z = s0 = s1 = 0;
 // rho1 = <b,b>, initial norm = ||b||.
@@ -144,9 +142,9 @@ __global__ __launch_bounds__(VNT) void k_upd1(const C<T>* __restrict__ r, const 
   const long long off = (long long)slot * n3;
   for (long long i = blockIdx.x * (long long)VNT + threadIdx.x; i < n3; i += (long long)nwg * VNT) {
     const long long j = off + i;
-    const C<T> sv = csub(r[j], smul<T>(al, v[j]));
-    s[j] = sv;
-    y[j] = csub(csub(sp[j], sv), smul<T>(al, w[j]));
+    C<T> yv;
+    s[j] = upd1_step<T>(al, r[j], v[j], sp[j], w[j], &yv);
+    y[j] = yv;
   }
 }
 
@@ -306,6 +304,15 @@ static int vec_nwg(long long n3) {
   return (int)std::max(1LL, std::min(w, 1024LL));
 }
 
+// PDDA_FUSE_UPD1=0: the first vector update runs as its own kernel.
+static bool upd1_fusion_off() {
+  static const bool off = [] {
+    const char* e = std::getenv("PDDA_FUSE_UPD1");
+    return e && std::string(e) == "0";
+  }();
+  return off;
+}
+
 template <typename T>
 int SolverT<T>::init(int nx_, int ny_, int nz_, int px_, int py_, int pz_, int nslots_, int precon_) {
   nx = nx_; ny = ny_; nz = nz_; nslots = nslots_; precon = precon_;
@@ -314,6 +321,7 @@ int SolverT<T>::init(int nx_, int ny_, int nz_, int px_, int py_, int pz_, int n
   const int prec = sizeof(T) == 8 ? 1 : 0;
   int rc;
   // planar plans: bind() gives them x-planar copies of the bound tables
+  mv.p1_upd = (!precon && !upd1_fusion_off()) ? 1 : 0;
   if ((rc = conv_plan_init(&mv, prec, nx, ny, nz, nx, ny, nz, nslots, 1))) return rc;
   if (precon && (rc = conv_plan_init(&pc, prec, nx, ny, nz, px, py, pz, nslots, 1))) return rc;
   nwg = vec_nwg(n3);
@@ -403,8 +411,10 @@ void SolverT<T>::advance_ages(int n) {
 
 template <typename T>
 int SolverT<T>::conv(ConvPlanBase* plan, int mode, const C<T>* x, C<T>* out, const C<T>* table, long long tstride,
-                     int dots, const C<T>* d0, const C<T>* d1, int need, hipStream_t s, bool control) {
+                     int dots, const C<T>* d0, const C<T>* d1, int need, hipStream_t s, bool control,
+                     const ConvArgs::Upd1* upd1) {
   ConvArgs a;
+  if (upd1) a.upd1 = *upd1;
   a.fin_control = control;
   a.mode = mode;
   a.x = x;
@@ -424,10 +434,16 @@ int SolverT<T>::conv(ConvPlanBase* plan, int mode, const C<T>* x, C<T>* out, con
 }
 
 template <typename T>
+bool SolverT<T>::fuse_upd1() const {
+  return !upd1_fusion_off() && !precon && conv_p1_fuses_upd1(&mv) && mv.g.vec_stride == n3;
+}
+
+template <typename T>
 int SolverT<T>::iterate(int n_iter, hipStream_t s) {
   if (!mask) { set_error("solver: operands not bound"); return PDDA_ERR_STATE; }
   int rc;
   const dim3 vg(nwg, nslots);
+  const bool fuse1 = fuse_upd1();
   for (int k = 0; k < n_iter; ++k) {
     C<T>* s_new = sidx ? vs.s0 : vs.s1;
     C<T>* s_prev = sidx ? vs.s1 : vs.s0;
@@ -439,14 +455,25 @@ int SolverT<T>::iterate(int n_iter, hipStream_t s) {
     }
     if ((rc = conv(&mv, MODE_INTERACT, ph, vs.v, ghat, ghat_stride, DOT_ALPHA, vs.rhat, nullptr, NEED_ACTIVE, s)))
       return rc;
-    hipLaunchKernelGGL(k_upd1<T>, vg, dim3(VNT), 0, s, vs.r, vs.v, s_new, s_prev, vs.w, vs.y, n3, ctl, nwg);
+    ConvArgs::Upd1 u1;
+    if (fuse1) {
+      u1.r = vs.r;
+      u1.v = vs.v;
+      u1.sp = s_prev;
+      u1.w = vs.w;
+      u1.s = s_new;
+      u1.y = vs.y;
+    } else {
+      hipLaunchKernelGGL(k_upd1<T>, vg, dim3(VNT), 0, s, vs.r, vs.v, s_new, s_prev, vs.w, vs.y, n3, ctl, nwg);
+    }
     const C<T>* sh = s_new;
     if (precon) {
       if ((rc = conv(&pc, MODE_PRECOND, s_new, vs.shat, mhat, mhat_stride, DOT_NONE, nullptr, nullptr, NEED_ACTIVE, s)))
         return rc;
       sh = vs.shat;
     }
-    if ((rc = conv(&mv, MODE_INTERACT, sh, vs.t, ghat, ghat_stride, DOT_ZETA, vs.y, s_new, NEED_ACTIVE, s)))
+    if ((rc = conv(&mv, MODE_INTERACT, sh, vs.t, ghat, ghat_stride, DOT_ZETA, vs.y, s_new, NEED_ACTIVE, s, false,
+                   fuse1 ? &u1 : nullptr)))
       return rc;
     hipLaunchKernelGGL(k_upd2<T>, vg, dim3(VNT), 0, s, vs, s_new, s_prev, vs.y, n3, partial, counters, ctl, nwg);
     const bool replace = (replace_mask(1) & 1) != 0;

@@ -120,4 +120,18 @@ __device__ inline bool slot_wanted(const SlotCtl* ctl, int slot, int need) {
   }
 }
 
+// Krylov scalar (complex<double>) times a vector element in the vector's precision.
+template <typename T> __device__ inline C<T> smul(cd s, C<T> a) {
+  return mk<T>((T)s.x * a.x - (T)s.y * a.y, (T)s.x * a.y + (T)s.y * a.x);
+}
+
+// First GPBiCG update of one element (compute_s / compute_y, Gpbicgstab.py:476-484):
+// s = r - alpha v ; y = s_prev - s - alpha w. Returns s, stores y.
+template <typename T>
+__device__ inline C<T> upd1_step(cd al, C<T> r, C<T> v, C<T> sp, C<T> w, C<T>* y) {
+  const C<T> s = csub(r, smul<T>(al, v));
+  *y = csub(csub(sp, s), smul<T>(al, w));
+  return s;
+}
+
 }  // namespace pdda

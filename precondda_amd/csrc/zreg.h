@@ -36,9 +36,23 @@ template <int L, bool INV = false> struct ZTile {
   static_assert(R >= 1 && NT % (3 * (P > Q ? P : Q)) == 0, "rows per block");
 };
 
-template <typename T, int L>
+// Operands of the first GPBiCG update when P1 computes its own input (UPD).
+template <typename T> struct ZUpd {
+  const C<T>* r;
+  const C<T>* v;
+  const C<T>* sp;
+  const C<T>* w;
+  C<T>* s;
+  C<T>* y;
+};
+
+// UPD: the input is s = r - alpha v of the slot (alpha = ctl[slot].alpha), computed here
+// with y = s_prev - s - alpha w; s and y are stored at the same box positions (every
+// position of the slot's box is loaded by exactly one thread), and x is not read.
+template <typename T, int L, bool UPD = false>
 __global__ __launch_bounds__(192) void kz_zfwd(const C<T>* __restrict__ x, C<T>* __restrict__ A, ConvGeom g,
-                                               const C<T>* __restrict__ twz, const SlotCtl* ctl, int need) {
+                                               const C<T>* __restrict__ twz, const SlotCtl* ctl, int need,
+                                               ZUpd<T> u) {
   using Z = ZTile<L>;
   constexpr int P = Z::P, Q = Z::Q, R = Z::R, RS = Z::RS, NT = Z::NT;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -56,11 +70,28 @@ __global__ __launch_bounds__(192) void kz_zfwd(const C<T>* __restrict__ x, C<T>*
   const bool ia = t < 3 * P * R;
   const int ca = t % 3, n1a = (t / 3) % P, ra = t / (3 * P);
   const bool rowa = ia && row0 + ra < rows;
-  const C<T>* src = x + slot * g.vec_stride + (row0 + ra) * 3LL * nz + ca;
+  const long long base = slot * g.vec_stride + (row0 + ra) * 3LL * nz + ca;
+  if constexpr (UPD) {
+    const cd al = ctl[slot].alpha;
 #pragma unroll
-  for (int n2 = 0; n2 < Q / 2; ++n2) {
-    const int pos = n1a + P * n2;
-    z[n2] = (rowa && pos < nz) ? src[pos * 3] : mk<T>(T(0), T(0));
+    for (int n2 = 0; n2 < Q / 2; ++n2) {
+      const int pos = n1a + P * n2;
+      z[n2] = mk<T>(T(0), T(0));
+      if (rowa && pos < nz) {
+        const long long j = base + pos * 3;
+        C<T> yv;
+        z[n2] = upd1_step<T>(al, u.r[j], u.v[j], u.sp[j], u.w[j], &yv);
+        u.s[j] = z[n2];
+        u.y[j] = yv;
+      }
+    }
+  } else {
+    const C<T>* src = x + base;
+#pragma unroll
+    for (int n2 = 0; n2 < Q / 2; ++n2) {
+      const int pos = n1a + P * n2;
+      z[n2] = (rowa && pos < nz) ? src[pos * 3] : mk<T>(T(0), T(0));
+    }
   }
   __syncthreads();  // twiddles
   if (ia) {

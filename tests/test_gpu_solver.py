@@ -43,7 +43,10 @@ def test_fp64_solve_matches_oracle(monkeypatch, precon, p3):
 def test_fp64_solve_register_z_passes(monkeypatch, precon):
     """The register z-inverse in every epilogue the solver launches: interaction with
     the alpha and zeta/eta dot products, the residual with its norms (replacement at
-    iteration 1) and without (reset recompute), and the preconditioner apply."""
+    iteration 1) and without (reset recompute), and the preconditioner apply. Without
+    the preconditioner the register z-forward of the second matvec also computes the
+    first vector update (s = r - alpha v, y = s_prev - s - alpha w); with it, that
+    update runs as its own kernel."""
     import precondda_amd as P
     monkeypatch.setenv("PDDA_Z_KERNEL", "register")
     pb, grid, k, ia, mask, G, M = _setup(4, 1.5 + 0.01j, 1.0)
