@@ -799,7 +799,11 @@ void fast_setup(ConvPlanBase* p) {
     }
     if (p3_forced() == 2 && xv.empty()) p->f_xv = 0;
   }
-  if (p->fast[0]) set_z_tiles(p, (z_forced() == 2 && zreg_rows(g.Lz) > 0) ? 3 : 0, p->f_rows);
+  // default z kernels: both register ones when forced; otherwise the LDS row tiles,
+  // except the register P1 of a plan whose P1 carries the solver's first vector update
+  int zr0 = 0;
+  if (zreg_rows(g.Lz) > 0) zr0 = z_forced() == 2 ? 3 : (p->p1_upd && z_forced() == 0) ? 1 : 0;
+  if (p->fast[0]) set_z_tiles(p, zr0, p->f_rows);
 }
 
 namespace {
