@@ -81,8 +81,9 @@ template <typename T> struct SolverT {
   int conv(ConvPlanBase* plan, int mode, const C<T>* x, C<T>* out, const C<T>* table, long long tstride, int dots,
            const C<T>* d0, const C<T>* d1, int need, hipStream_t s, bool control = false,
            const ConvArgs::Upd1* upd1 = nullptr);
-  // True when the first vector update runs inside the second matvec's P1 (no
-  // preconditioner, register z-forward pass; PDDA_FUSE_UPD1=0 turns it off).
+  // True when the first vector update runs inside the P1 that first reads s: the
+  // preconditioner apply's, else the second matvec's (register z-forward pass;
+  // PDDA_FUSE_UPD1=0 turns it off).
   bool fuse_upd1() const;
 };
 

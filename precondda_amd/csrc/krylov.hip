@@ -321,7 +321,9 @@ int SolverT<T>::init(int nx_, int ny_, int nz_, int px_, int py_, int pz_, int n
   const int prec = sizeof(T) == 8 ? 1 : 0;
   int rc;
   // planar plans: bind() gives them x-planar copies of the bound tables
+  // the plan whose P1 first reads s (the preconditioner's when there is one) computes it
   mv.p1_upd = (!precon && !upd1_fusion_off()) ? 1 : 0;
+  pc.p1_upd = (precon && !upd1_fusion_off()) ? 1 : 0;
   if ((rc = conv_plan_init(&mv, prec, nx, ny, nz, nx, ny, nz, nslots, 1))) return rc;
   if (precon && (rc = conv_plan_init(&pc, prec, nx, ny, nz, px, py, pz, nslots, 1))) return rc;
   nwg = vec_nwg(n3);
@@ -435,7 +437,8 @@ int SolverT<T>::conv(ConvPlanBase* plan, int mode, const C<T>* x, C<T>* out, con
 
 template <typename T>
 bool SolverT<T>::fuse_upd1() const {
-  return !upd1_fusion_off() && !precon && conv_p1_fuses_upd1(&mv) && mv.g.vec_stride == n3;
+  const ConvPlanBase& q = precon ? pc : mv;
+  return !upd1_fusion_off() && conv_p1_fuses_upd1(&q) && q.g.vec_stride == n3;
 }
 
 template <typename T>
@@ -468,12 +471,13 @@ int SolverT<T>::iterate(int n_iter, hipStream_t s) {
     }
     const C<T>* sh = s_new;
     if (precon) {
-      if ((rc = conv(&pc, MODE_PRECOND, s_new, vs.shat, mhat, mhat_stride, DOT_NONE, nullptr, nullptr, NEED_ACTIVE, s)))
+      if ((rc = conv(&pc, MODE_PRECOND, s_new, vs.shat, mhat, mhat_stride, DOT_NONE, nullptr, nullptr, NEED_ACTIVE, s,
+                     false, fuse1 ? &u1 : nullptr)))
         return rc;
       sh = vs.shat;
     }
     if ((rc = conv(&mv, MODE_INTERACT, sh, vs.t, ghat, ghat_stride, DOT_ZETA, vs.y, s_new, NEED_ACTIVE, s, false,
-                   fuse1 ? &u1 : nullptr)))
+                   fuse1 && !precon ? &u1 : nullptr)))
       return rc;
     hipLaunchKernelGGL(k_upd2<T>, vg, dim3(VNT), 0, s, vs, s_new, s_prev, vs.y, n3, partial, counters, ctl, nwg);
     const bool replace = (replace_mask(1) & 1) != 0;

@@ -44,11 +44,27 @@ def test_fp64_solve_register_z_passes(monkeypatch, precon):
     """The register z-inverse in every epilogue the solver launches: interaction with
     the alpha and zeta/eta dot products, the residual with its norms (replacement at
     iteration 1) and without (reset recompute), and the preconditioner apply. Without
-    the preconditioner the register z-forward of the second matvec also computes the
-    first vector update (s = r - alpha v, y = s_prev - s - alpha w); with it, that
-    update runs as its own kernel."""
+    the first vector update (s = r - alpha v, y = s_prev - s - alpha w) is computed
+    by the register z-forward that first reads s: the preconditioner apply's, or the
+    second matvec's without the preconditioner."""
     import precondda_amd as P
     monkeypatch.setenv("PDDA_Z_KERNEL", "register")
+    pb, grid, k, ia, mask, G, M = _setup(4, 1.5 + 0.01j, 1.0)
+    ref = O.solve_gpbicgstab(grid, ia, G, M, pb["E_inc"], mask, 100, 300, 1e-10, precon=precon)
+    out = P.solve_gpbicgstab(grid, ia, torch.as_tensor(G).cuda(), torch.as_tensor(M).cuda(), pb["E_inc"], mask,
+                             100, 300, 1e-10, precon=precon)
+    print("iters", ref[0], out[0])
+    assert out[0] == ref[0]
+    err = np.linalg.norm(out[1].cpu().numpy() - ref[1]) / np.linalg.norm(ref[1])
+    print("P rel err", err)
+    assert err < 1e-10
+
+
+@pytest.mark.parametrize("precon", [True, False])
+def test_fp64_solve_stockham_z_passes(monkeypatch, precon):
+    """LDS-tiled z passes: the first vector update runs as its own kernel (k_upd1)."""
+    import precondda_amd as P
+    monkeypatch.setenv("PDDA_Z_KERNEL", "stockham")
     pb, grid, k, ia, mask, G, M = _setup(4, 1.5 + 0.01j, 1.0)
     ref = O.solve_gpbicgstab(grid, ia, G, M, pb["E_inc"], mask, 100, 300, 1e-10, precon=precon)
     out = P.solve_gpbicgstab(grid, ia, torch.as_tensor(G).cuda(), torch.as_tensor(M).cuda(), pb["E_inc"], mask,
